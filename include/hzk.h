@@ -1,0 +1,101 @@
+/* hzk.h — C ABI of the MI355X-native alt_bn128 Groth16 prover (libhzk.so).
+ *
+ * The drop-in boundary for LibLightZK's operator and protocol API: every entry
+ * point below names the reference interface it replaces (file:line in the
+ * reference tree). Plain C types only; device buffers are `void*` device
+ * pointers owned by the caller; results are valid after hzk_sync().
+ *
+ * Byte layouts (little-endian, Montgomery form, R = 2^256, fully reduced):
+ *   Fr / Fq element   32 B  (= reference mont_t / libff bigint<4>)
+ *   G1 affine         64 B  x | y                 infinity = 64 zero bytes
+ *   G2 affine        128 B  x.c0 | x.c1 | y.c0 | y.c1   infinity = zero bytes
+ *   G1 Jacobian       96 B  X | Y | Z              (reference g1_t, jacobian_t.cuh:11)
+ *   G2 Jacobian      192 B  X.c0|X.c1|Y.c0|Y.c1|Z.c0|Z.c1 (reference write_to, alt_bn128_t.cuh:252-256)
+ * MSM results are written normalised: (x, y, 1), or (0, 1, 0) for infinity
+ * (libff's convention, alt_bn128_g1.cu:56-73).
+ *
+ * Return codes: 0 on success, a negative HZK_ERR_* otherwise; the message is
+ * in hzk_last_error(). No entry point aborts the process.
+ */
+#ifndef HZK_H
+#define HZK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HZK_OK 0
+#define HZK_ERR_INVALID (-1)   /* bad argument */
+#define HZK_ERR_NOMEM (-2)     /* device or host allocation failed */
+#define HZK_ERR_HIP (-3)       /* HIP runtime error */
+#define HZK_ERR_UNSUPPORTED (-4)
+
+typedef struct hzk_ctx hzk_ctx;
+
+/* ---- context (replaces the reference's implicit CUDA runtime state and legacy
+ * default stream, include/device-arith/device-utils.cuh:3) ---- */
+int hzk_ctx_create(int device, void* hip_stream /* nullable: own stream */, hzk_ctx** out);
+void hzk_ctx_destroy(hzk_ctx* ctx);
+const char* hzk_last_error(const hzk_ctx* ctx);
+int hzk_sync(hzk_ctx* ctx);
+void* hzk_ctx_stream(hzk_ctx* ctx);
+const char* hzk_version(void);
+
+/* ---- buffers (replaces xpu::vector<T> allocate/store/load/clear,
+ * include/data-structure/xpu-vector.cuh:10-101) ---- */
+int hzk_device_alloc(hzk_ctx* ctx, size_t bytes, void** out);
+int hzk_device_free(hzk_ctx* ctx, void* ptr);
+int hzk_memcpy_h2d(hzk_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int hzk_memcpy_d2h(hzk_ctx* ctx, void* dst, const void* src, size_t bytes);   /* synchronous */
+int hzk_memset(hzk_ctx* ctx, void* dst, int value, size_t bytes);              /* stream-ordered */
+int hzk_mem_info(hzk_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* ---- device field arithmetic unit entry (the reference exercises mont_t/fp2_t
+ * through test kernels: test/fp2-check.cu:7-62, test/cpu-gpu-check.cu:15-57).
+ * field: 0 Fq, 1 Fr, 2 Fq2. op: 0 mul, 1 add, 2 sub, 3 neg(a), 4 sqr(a),
+ * 5 inv(a), 6 from_mont(a), 7 to_mont(a). a, b, out: device arrays of n
+ * elements. ---- */
+int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, void* out, size_t n);
+
+/* ---- MSM (replaces multi_scalar_multiplication_g1/_g2,
+ * include/operator/msm.cuh:210-248).
+ * scalars: device Fr[n] (Montgomery); bases: device affine points (layout
+ * HZK_BASES_AFFINE); dest: device pointer to one Jacobian point (96 / 192 B),
+ * written normalised; dest may alias other buffers of the call.
+ * win_siz: window bits c (0 = autotuned). ---- */
+#define HZK_BASES_AFFINE 0u
+int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
+               size_t win_siz, void* dest);
+int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
+               size_t win_siz, void* dest);
+
+/* ---- timing (replaces the reference's Timer phase printouts,
+ * include/timer/timer.cuh:5-12). When enabled, each kernel launch is
+ * bracketed by HIP events on the context stream; totals are per kernel name. */
+int hzk_timing_enable(hzk_ctx* ctx, int enable);
+int hzk_timing_reset(hzk_ctx* ctx);
+/* total milliseconds and launch count of kernel `name` since the last reset */
+int hzk_timing_get(hzk_ctx* ctx, const char* name, double* ms, long* launches);
+/* newline-separated "name ms launches" report; returns bytes needed */
+size_t hzk_timing_report(hzk_ctx* ctx, char* buf, size_t cap);
+
+/* ---- roofline denominators (no counterpart in the reference: the measured
+ * peaks that achieved rates are divided by). kind: 0 v_mad_u64_u32 per second,
+ * 1 Fq Montgomery products per second, 2 32-bit VALU adds per second,
+ * 3 HBM copy bytes per second (read + write). */
+int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value);
+
+/* ---- tuning / test hooks ---- */
+/* force MSM window bits and bucket-reduction segment size (0 = automatic) */
+int hzk_set_tuning(hzk_ctx* ctx, int window_bits, int segment);
+/* make the n-th following scratch allocation fail with HZK_ERR_NOMEM (-1 = off) */
+int hzk_inject_alloc_failure(hzk_ctx* ctx, long after_n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HZK_H */

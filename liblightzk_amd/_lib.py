@@ -1,0 +1,66 @@
+"""Loader for the in-tree C ABI library liblightzk_amd/libhzk.so (include/hzk.h).
+
+The library is required: there is no CPU fallback anywhere in this package,
+so a missing or unloadable library raises immediately.
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhzk.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "hzk.h")
+
+_lib = None
+
+
+class HzkError(RuntimeError):
+    pass
+
+
+def header_functions(path=HEADER):
+    """Names of every function the C ABI header declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hzk_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HzkError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C liblightzk_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, u32, i64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32, ctypes.c_long
+    dbl = ctypes.c_double
+    sigs = {
+        "hzk_ctx_create": (i32, [i32, vp, ctypes.POINTER(vp)]),
+        "hzk_ctx_destroy": (None, [vp]),
+        "hzk_last_error": (ctypes.c_char_p, [vp]),
+        "hzk_sync": (i32, [vp]),
+        "hzk_ctx_stream": (vp, [vp]),
+        "hzk_version": (ctypes.c_char_p, []),
+        "hzk_device_alloc": (i32, [vp, sz, ctypes.POINTER(vp)]),
+        "hzk_device_free": (i32, [vp, vp]),
+        "hzk_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "hzk_memcpy_d2h": (i32, [vp, vp, vp, sz]),
+        "hzk_memset": (i32, [vp, vp, i32, sz]),
+        "hzk_mem_info": (i32, [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+        "hzk_field_op": (i32, [vp, i32, i32, vp, vp, vp, sz]),
+        "hzk_msm_g1": (i32, [vp, vp, vp, sz, u32, sz, vp]),
+        "hzk_msm_g2": (i32, [vp, vp, vp, sz, u32, sz, vp]),
+        "hzk_timing_enable": (i32, [vp, i32]),
+        "hzk_timing_reset": (i32, [vp]),
+        "hzk_timing_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]),
+        "hzk_timing_report": (sz, [vp, ctypes.c_char_p, sz]),
+        "hzk_set_tuning": (i32, [vp, i32, i32]),
+        "hzk_roofline_probe": (i32, [vp, i32, ctypes.POINTER(dbl)]),
+        "hzk_inject_alloc_failure": (i32, [vp, i64]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,305 @@
+// C ABI: context, buffers, timing, the device field-arithmetic unit entry,
+// and the MSM entry points (declarations and reference citations in
+// include/hzk.h).
+#include <cstring>
+#include <mutex>
+#include "common.hpp"
+#include "curve.hpp"
+#include "msm.hpp"
+
+namespace hzk {
+
+int set_error(hzk_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->last_error = buf;
+    return code;
+}
+
+int ensure_scratch(hzk_ctx* ctx, size_t bytes) {
+    if (ctx->fail_alloc_after >= 0) {
+        if (ctx->fail_alloc_after == 0) {
+            ctx->fail_alloc_after = -1;
+            return set_error(ctx, HZK_ERR_NOMEM, "scratch allocation of %zu bytes failed (injected)", bytes);
+        }
+        --ctx->fail_alloc_after;
+    }
+    if (ctx->scratch_bytes >= bytes) return 0;
+    if (ctx->scratch) {
+        HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        HZK_HIP(ctx, hipFree(ctx->scratch));
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&ctx->scratch, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        e = hipMalloc(&ctx->scratch, bytes);
+        want = bytes;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->scratch = nullptr;
+        return set_error(ctx, HZK_ERR_NOMEM, "scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    ctx->scratch_bytes = want;
+    return 0;
+}
+
+int ensure_pinned(hzk_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes >= bytes) return 0;
+    if (ctx->pinned) {
+        HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        HZK_HIP(ctx, hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+    }
+    size_t want = bytes < 65536 ? 65536 : bytes;
+    HZK_HIP(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+    ctx->pinned_bytes = want;
+    return 0;
+}
+
+static hipEvent_t take_event(hzk_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+static void drain_timing(hzk_ctx* ctx) {
+    for (auto& r : ctx->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            auto& t = ctx->totals[r.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        ctx->event_pool.push_back(r.a);
+        ctx->event_pool.push_back(r.b);
+    }
+    ctx->pending.clear();
+}
+
+TimedLaunch::TimedLaunch(hzk_ctx* c, const char* name) : ctx(c) {
+    if (!ctx->timing) return;
+    hipEvent_t a = take_event(ctx);
+    b = take_event(ctx);
+    if (!a || !b) { b = nullptr; return; }
+    (void)hipEventRecord(a, ctx->stream);
+    ctx->pending.push_back({a, b, name});
+}
+TimedLaunch::~TimedLaunch() {
+    if (b) (void)hipEventRecord(b, ctx->stream);
+}
+
+// ------------------------------------------------------------------ field unit kernels
+template <class F>
+__global__ void k_field_op(int op, const F* a, const F* b, F* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    F x = a[i], r;
+    switch (op) {
+        case 0: r = x * b[i]; break;
+        case 1: r = x + b[i]; break;
+        case 2: r = x - b[i]; break;
+        case 3: r = x.neg(); break;
+        case 4: r = x.sqr(); break;
+        case 5: r = x.inverse(); break;
+        default: r = x; break;
+    }
+    out[i] = r;
+}
+template <class F>
+__global__ void k_field_mont(int op, const F* a, F* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = op == 6 ? a[i].from_mont() : a[i].to_mont();
+}
+
+}  // namespace hzk
+
+using namespace hzk;
+
+extern "C" {
+
+const char* hzk_version(void) { return "hzk 0.1 gfx950"; }
+
+int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
+    if (!out) return HZK_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HZK_ERR_HIP;
+    if (device < 0 || device >= ndev) return HZK_ERR_INVALID;
+    hzk_ctx* ctx = new hzk_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
+    if (stream) {
+        ctx->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
+        ctx->own_stream = true;
+    }
+    *out = ctx;
+    return HZK_OK;
+}
+
+void hzk_ctx_destroy(hzk_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_timing(ctx);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* hzk_last_error(const hzk_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+void* hzk_ctx_stream(hzk_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int hzk_sync(hzk_ctx* ctx) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HZK_OK;
+}
+
+int hzk_device_alloc(hzk_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return HZK_ERR_INVALID;
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return set_error(ctx, HZK_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return HZK_OK;
+}
+int hzk_device_free(hzk_ctx* ctx, void* ptr) {
+    if (!ctx) return HZK_ERR_INVALID;
+    if (!ptr) return HZK_OK;
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HZK_HIP(ctx, hipFree(ptr));
+    return HZK_OK;
+}
+int hzk_memcpy_h2d(hzk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HZK_OK;
+}
+int hzk_memcpy_d2h(hzk_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HZK_OK;
+}
+int hzk_memset(hzk_ctx* ctx, void* dst, int value, size_t bytes) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return HZK_OK;
+}
+int hzk_mem_info(hzk_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipMemGetInfo(free_bytes, total_bytes));
+    return HZK_OK;
+}
+
+int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, void* out, size_t n) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, field >= 0 && field <= 2 && op >= 0 && op <= 7, HZK_ERR_INVALID, "field_op: bad field/op");
+    HZK_CHECK(ctx, a && out && (b || op >= 3), HZK_ERR_INVALID, "field_op: null pointer");
+    HZK_CHECK(ctx, !(field == 2 && op >= 6), HZK_ERR_UNSUPPORTED, "field_op: Fq2 has no Montgomery conversion");
+    if (n == 0) return HZK_OK;
+    dim3 grid(blocks_for(n, 256)), block(256);
+    TimedLaunch tl(ctx, "field_op");
+    if (op >= 6) {
+        if (field == 0) hipLaunchKernelGGL(k_field_mont<fq_t>, grid, block, 0, ctx->stream, op, (const fq_t*)a, (fq_t*)out, n);
+        else hipLaunchKernelGGL(k_field_mont<fr_t>, grid, block, 0, ctx->stream, op, (const fr_t*)a, (fr_t*)out, n);
+    } else if (field == 0) {
+        hipLaunchKernelGGL(k_field_op<fq_t>, grid, block, 0, ctx->stream, op, (const fq_t*)a, (const fq_t*)b, (fq_t*)out, n);
+    } else if (field == 1) {
+        hipLaunchKernelGGL(k_field_op<fr_t>, grid, block, 0, ctx->stream, op, (const fr_t*)a, (const fr_t*)b, (fr_t*)out, n);
+    } else {
+        hipLaunchKernelGGL(k_field_op<fq2_t>, grid, block, 0, ctx->stream, op, (const fq2_t*)a, (const fq2_t*)b, (fq2_t*)out, n);
+    }
+    HZK_HIP(ctx, hipGetLastError());
+    return HZK_OK;
+}
+
+int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
+               void* dest) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g1: base layout %u unsupported", base_layout);
+    return msm_g1(ctx, scalars, bases, n, win_siz, dest);
+}
+int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
+               void* dest) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
+    return msm_g2(ctx, scalars, bases, n, win_siz, dest);
+}
+
+int hzk_timing_enable(hzk_ctx* ctx, int enable) {
+    if (!ctx) return HZK_ERR_INVALID;
+    ctx->timing = enable != 0;
+    return HZK_OK;
+}
+int hzk_timing_reset(hzk_ctx* ctx) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drain_timing(ctx);
+    ctx->totals.clear();
+    return HZK_OK;
+}
+int hzk_timing_get(hzk_ctx* ctx, const char* name, double* ms, long* launches) {
+    if (!ctx || !name) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    drain_timing(ctx);
+    auto it = ctx->totals.find(name);
+    if (ms) *ms = it == ctx->totals.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == ctx->totals.end() ? 0 : it->second.second;
+    return HZK_OK;
+}
+size_t hzk_timing_report(hzk_ctx* ctx, char* buf, size_t cap) {
+    if (!ctx) return 0;
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_timing(ctx);
+    std::string s;
+    for (auto& kv : ctx->totals) {
+        char line[256];
+        snprintf(line, sizeof line, "%s %.6f %ld\n", kv.first.c_str(), kv.second.first, kv.second.second);
+        s += line;
+    }
+    if (buf && cap) {
+        size_t m = s.size() < cap - 1 ? s.size() : cap - 1;
+        memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return s.size() + 1;
+}
+
+int hzk_set_tuning(hzk_ctx* ctx, int window_bits, int segment) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, window_bits == 0 || (window_bits >= MSM_MIN_C && window_bits <= MSM_MAX_C), HZK_ERR_INVALID,
+              "window_bits %d out of range", window_bits);
+    HZK_CHECK(ctx, segment == 0 || (segment > 0 && (segment & (segment - 1)) == 0), HZK_ERR_INVALID,
+              "segment %d must be a power of two", segment);
+    ctx->force_window = window_bits;
+    ctx->force_seg = segment;
+    return HZK_OK;
+}
+int hzk_inject_alloc_failure(hzk_ctx* ctx, long after_n) {
+    if (!ctx) return HZK_ERR_INVALID;
+    ctx->fail_alloc_after = after_n;
+    return HZK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Library context: device, stream, scratch arena, per-kernel timing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+#include "../../include/hzk.h"
+
+struct hzk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    // scratch arena: grown on demand, reused across calls, freed at destroy
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // pinned host staging for small results
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // timing: when enabled every kernel launch is bracketed by events
+    bool timing = false;
+    struct Rec { hipEvent_t a, b; std::string name; };
+    std::vector<Rec> pending;
+    std::map<std::string, std::pair<double, long>> totals;   // name -> (ms, launches)
+    std::vector<hipEvent_t> event_pool;
+    // fault injection for tests of the error paths (host-side only)
+    long fail_alloc_after = -1;
+    // autotune overrides (0 = automatic)
+    int force_window = 0;
+    int force_seg = 0;
+};
+
+namespace hzk {
+
+int set_error(hzk_ctx* ctx, int code, const char* fmt, ...);
+
+#define HZK_HIP(ctx, expr)                                                            \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess)                                                         \
+            return ::hzk::set_error((ctx), HZK_ERR_HIP, "%s failed: %s (%s:%d)", #expr, \
+                                    hipGetErrorString(_e), __FILE__, __LINE__);       \
+    } while (0)
+
+#define HZK_CHECK(ctx, cond, code, ...)                                   \
+    do {                                                                  \
+        if (!(cond)) return ::hzk::set_error((ctx), (code), __VA_ARGS__); \
+    } while (0)
+
+// Scratch arena bump allocator for one operation (256-byte aligned slices).
+struct Arena {
+    char* base;
+    size_t size, used = 0;
+    void* take(size_t bytes) {
+        size_t off = (used + 255) & ~size_t(255);
+        used = off + bytes;
+        return base + off;
+    }
+};
+// Make sure ctx->scratch holds at least `bytes`; returns 0 or an error code.
+int ensure_scratch(hzk_ctx* ctx, size_t bytes);
+int ensure_pinned(hzk_ctx* ctx, size_t bytes);
+
+// Kernel timing helpers (no-ops unless ctx->timing).
+struct TimedLaunch {
+    hzk_ctx* ctx;
+    hipEvent_t b = nullptr;
+    TimedLaunch(hzk_ctx* c, const char* name);
+    ~TimedLaunch();
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace hzk

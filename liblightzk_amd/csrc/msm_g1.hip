@@ -1,0 +1,8 @@
+// G1 instantiation of the MSM driver (kernels in msm_impl.hpp).
+#include "msm_impl.hpp"
+
+namespace hzk {
+int msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest) {
+    return msm_run<fq_t, host::G1x>(ctx, scalars, bases, n, win_siz, dest);
+}
+}  // namespace hzk

@@ -26,3 +26,11 @@ def golden():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "reference_constants.json")) as f:
         return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    import liblightzk_amd as z
+    c = z.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,38 @@
+"""CPU-side checks of the C ABI boundary: the library loads without a GPU and
+exports every function include/hzk.h declares (no compute calls here)."""
+import ctypes
+import os
+
+import pytest
+
+import liblightzk_amd
+from liblightzk_amd import _lib
+
+
+def test_library_present_and_exports_header():
+    assert os.path.exists(_lib.LIB_PATH), "libhzk.so not built"
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _lib.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_loader_binds_every_header_function():
+    L = _lib.load()
+    for n in _lib.header_functions():
+        assert getattr(L, n).argtypes is not None, f"{n} has no ctypes signature"
+
+
+def test_version_string_without_gpu():
+    assert b"gfx950" in _lib.load().hzk_version()
+
+
+def test_context_creation_fails_cleanly_without_gpu():
+    import subprocess, sys
+    # probe GPU presence in a child so this process never initialises HIP
+    code = "import ctypes,sys; sys.path.insert(0, %r); import liblightzk_amd as z\n" \
+           "try:\n  z.Context(0)\n  print('gpu')\nexcept z.HzkError as e:\n  print('nogpu')\n" % os.path.dirname(
+               os.path.dirname(os.path.abspath(liblightzk_amd.__file__)))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.stdout.strip() in ("gpu", "nogpu"), out.stderr

@@ -1,0 +1,133 @@
+"""GPU Pippenger MSM (hzk_msm_g1 / hzk_msm_g2) vs the CPU oracle, bit-exact.
+
+Semantics of the reference operator (include/operator/msm.cuh:210-248, checked
+there by test/msm-check.cu:43-57 against libff): dest = sum s_i * P_i, written
+as a normalised Jacobian point. Pinned by the oracle's Pippenger, its naive
+double-and-add sum and the closed form (sum s_i k_i) G for P_i = k_i G; edge
+cases: n = 1, all-zero and all-one scalars, repeated bases (doubling inside a
+bucket), infinity bases, P + (-P), the 70%-{0,1} sparse witness (one hot
+bucket, multi-level segment merge) and every supported window size.
+"""
+import numpy as np
+import pytest
+
+import pymodel as pm
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(orc, seed, n, sparse=False):
+    s = orc.gen_fr(seed, 0, n, sparse=sparse)
+    k = orc.gen_fr(seed, 1, n)
+    return s, k
+
+
+def _run_g1(ctx, s, bases, win=0):
+    n = s.size // 4
+    ds, db, dd = ctx.upload(s), ctx.upload(bases), ctx.alloc(96)
+    ctx.msm_g1(ds, db, n, dd, win)
+    return dd.download(np.uint64, 12)
+
+
+def _run_g2(ctx, s, bases, win=0):
+    n = s.size // 4
+    ds, db, dd = ctx.upload(s), ctx.upload(bases), ctx.alloc(192)
+    ctx.msm_g2(ds, db, n, dd, win)
+    return dd.download(np.uint64, 24)
+
+
+@pytest.fixture(scope="module")
+def g1_data(orc):
+    n = 1 << 14
+    s, k = _case(orc, 1, n)
+    return s, k, orc.g1_mul_gen(k)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 1000, 1 << 14])
+def test_g1_random_vs_oracle(ctx, orc, g1_data, n):
+    s, k, bases = g1_data
+    s, k, bases = s[: 4 * n].copy(), k[: 4 * n].copy(), bases[: 8 * n].copy()
+    got = _run_g1(ctx, s, bases)
+    assert np.array_equal(got, orc.msm_g1(s, bases, 1)), n
+    assert np.array_equal(got, orc.msm_closed_form(s, k, 1)), n
+
+
+@pytest.mark.parametrize("win", [4, 5, 8, 11, 12, 13, 16, 20])
+def test_g1_every_window(ctx, orc, g1_data, win):
+    s, k, bases = g1_data
+    n = 5000
+    s, k, bases = s[: 4 * n].copy(), k[: 4 * n].copy(), bases[: 8 * n].copy()
+    assert np.array_equal(_run_g1(ctx, s, bases, win), orc.msm_closed_form(s, k, 1)), win
+
+
+def test_g1_special_scalars_and_bases(ctx, orc, g1_data):
+    s, k, bases = g1_data
+    n = 3000
+    bases = bases[: 8 * n].copy()
+    zero = np.zeros(4 * n, dtype=np.uint64)
+    got = _run_g1(ctx, zero, bases)
+    assert pm.g1_jac_from_buf(got) is None
+    assert np.array_equal(got, orc.msm_g1(zero, bases, 1))   # libff infinity (0, 1, 0)
+    one = np.tile(pm.int_to_limbs(pm.MONT % pm.R), n)
+    assert np.array_equal(_run_g1(ctx, one, bases), orc.msm_g1(one, bases, 1))
+    # repeated bases, a few infinity bases, P and -P in the same MSM
+    rep = np.tile(bases[: 8 * 7], n // 7 + 1)[: 8 * n].copy()
+    rep[8 * 3: 8 * 4] = 0
+    neg = pm.g1_to_aff_buf(pm.G1C.neg(pm.g1_aff_from_buf(rep[:8])))
+    rep[8 * 10: 8 * 11] = neg
+    sc = s[: 4 * n].copy()
+    sc[4 * 10: 4 * 11] = sc[0:4]   # s*P + s*(-P) cancels
+    for win in (0, 4, 9):
+        assert np.array_equal(_run_g1(ctx, sc, rep, win), orc.msm_g1(sc, rep, 1)), win
+
+
+def test_g1_sparse_hot_bucket(ctx, orc):
+    n = 1 << 15
+    s, k = _case(orc, 3, n, sparse=True)
+    bases = orc.g1_mul_gen(k)
+    want = orc.msm_closed_form(s, k, 1)
+    for win in (0, 4, 8):   # c = 4: ~2^15/8 entries per bucket -> several merge levels
+        assert np.array_equal(_run_g1(ctx, s, bases, win), want), win
+
+
+def test_g1_deterministic(ctx, orc, g1_data):
+    s, k, bases = g1_data
+    a = _run_g1(ctx, s, bases)
+    b = _run_g1(ctx, s, bases)
+    assert np.array_equal(a, b)
+
+
+def test_g1_naive_semantics_small(ctx, orc):
+    """test/msm-check.cu:46-55: per-thread s_i * P_i accumulators (oracle method 0)."""
+    s, k = _case(orc, 9, 300)
+    bases = orc.g1_mul_gen(k)
+    assert np.array_equal(_run_g1(ctx, s, bases), orc.msm_g1(s, bases, 0))
+
+
+@pytest.mark.parametrize("n", [1, 50, 777])
+def test_g2_vs_oracle(ctx, orc, n):
+    s, k = _case(orc, 4, n)
+    bases = orc.g2_mul_gen(k)
+    got = _run_g2(ctx, s, bases)
+    assert np.array_equal(got, orc.msm_g2(s, bases, 1)), n
+    assert np.array_equal(got, orc.msm_closed_form(s, k, 2)), n
+
+
+def test_g2_special(ctx, orc):
+    n = 400
+    s, k = _case(orc, 5, n, sparse=True)
+    bases = orc.g2_mul_gen(k)
+    bases[16 * 7: 16 * 8] = 0
+    rep = bases.copy()
+    rep[16 * 20: 16 * 40] = np.tile(bases[16 * 19: 16 * 20], 20)
+    for win in (0, 4, 7):
+        assert np.array_equal(_run_g2(ctx, s, rep, win), orc.msm_g2(s, rep, 1)), win
+
+
+def test_msm_rejects_bad_arguments(ctx):
+    from liblightzk_amd import HzkError
+    d = ctx.alloc(4096)
+    with pytest.raises(HzkError):
+        ctx.msm_g1(d, d, 0, d)
+    with pytest.raises(HzkError):
+        ctx.msm_g1(d, d, 4, d, win_siz=21)
