@@ -72,6 +72,23 @@ int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
                size_t win_siz, void* dest);
 
+/* statistics of the last MSM on this context: {bucket entries (nonzero
+ * digits), accumulation segments, largest bucket, window bits c, windows} */
+int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5);
+
+/* ---- fixed-base generation (replaces fix_base_multi_scalar_multiplication_g1/_g2,
+ * include/operator/msm.cuh:5-46, plus the affine normalisation of pre_comp_g1/_g2,
+ * msm.cuh:48-75): out_affine[i] = k_i * G for device Fr scalars k (Montgomery);
+ * scalars are not modified (the reference converts them in place, msm.cuh:16). */
+int hzk_fixbase_g1(hzk_ctx* ctx, const void* scalars, size_t n, void* out_affine);
+int hzk_fixbase_g2(hzk_ctx* ctx, const void* scalars, size_t n, void* out_affine);
+
+/* ---- seeded synthetic inputs (replaces the reference's std::random_device draws,
+ * include/host-arith/field/bigint.cuh:172-182, and Fp_model::random_element,
+ * include/host-arith/field/fp.cuh:789-816). kind: 0 uniform Fr, 1 sparse Fr
+ * (70% in {0,1}), 2 uniform Fq; n elements of 32 B into host memory. */
+int hzk_gen_field(uint64_t seed, uint64_t stream, size_t n, int kind, void* host_out);
+
 /* ---- timing (replaces the reference's Timer phase printouts,
  * include/timer/timer.cuh:5-12). When enabled, each kernel launch is
  * bracketed by HIP events on the context stream; totals are per kernel name. */

@@ -171,11 +171,32 @@ class Context:
         self._check(load().hzk_field_op(self.handle, fields[field], ops[op], int(a), int(b) if b is not None else None,
                                         int(out), n))
 
+    def msm_last_stats(self):
+        a = (ctypes.c_uint32 * 5)()
+        self._check(load().hzk_msm_last_stats(self.handle, a))
+        return dict(zip(("entries", "segments", "max_bucket", "c", "windows"), list(a)))
+
+    def fixbase_g1(self, scalars, n, out_affine):
+        self._check(load().hzk_fixbase_g1(self.handle, int(scalars), n, int(out_affine)))
+
+    def fixbase_g2(self, scalars, n, out_affine):
+        self._check(load().hzk_fixbase_g2(self.handle, int(scalars), n, int(out_affine)))
+
     def msm_g1(self, scalars, bases, n, dest, win_siz=0):
         self._check(load().hzk_msm_g1(self.handle, int(scalars), int(bases), n, 0, win_siz, int(dest)))
 
     def msm_g2(self, scalars, bases, n, dest, win_siz=0):
         self._check(load().hzk_msm_g2(self.handle, int(scalars), int(bases), n, 0, win_siz, int(dest)))
+
+
+def gen_field(seed, stream, n, kind="fr"):
+    """Seeded synthetic field elements (host numpy uint64, 4 limbs each)."""
+    out = np.empty(4 * n, dtype=np.uint64)
+    code = {"fr": 0, "fr_sparse": 1, "fq": 2}[kind]
+    rc = load().hzk_gen_field(seed, stream, n, code, out.ctypes.data)
+    if rc != 0:
+        raise HzkError(f"hzk_gen_field rc={rc}")
+    return out
 
 
 def multi_scalar_multiplication_g1(ctx, length, scalars, points, dest, win_siz=0, win_cnt=None, bucket_cnt=None,

@@ -50,6 +50,10 @@ def load():
         "hzk_field_op": (i32, [vp, i32, i32, vp, vp, vp, sz]),
         "hzk_msm_g1": (i32, [vp, vp, vp, sz, u32, sz, vp]),
         "hzk_msm_g2": (i32, [vp, vp, vp, sz, u32, sz, vp]),
+        "hzk_msm_last_stats": (i32, [vp, ctypes.POINTER(ctypes.c_uint32)]),
+        "hzk_fixbase_g1": (i32, [vp, vp, sz, vp]),
+        "hzk_fixbase_g2": (i32, [vp, vp, sz, vp]),
+        "hzk_gen_field": (i32, [ctypes.c_uint64, ctypes.c_uint64, sz, i32, vp]),
         "hzk_timing_enable": (i32, [vp, i32]),
         "hzk_timing_reset": (i32, [vp]),
         "hzk_timing_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]),

@@ -141,6 +141,8 @@ int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
     hzk_ctx* ctx = new hzk_ctx();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->lds_per_block = prop.sharedMemPerBlock;
     if (stream) {
         ctx->stream = (hipStream_t)stream;
     } else {
@@ -158,6 +160,8 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     drain_timing(ctx);
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);
+    if (ctx->comb_g2) (void)hipFree(ctx->comb_g2);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -245,6 +249,12 @@ int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
     if (!ctx) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
     return msm_g2(ctx, scalars, bases, n, win_siz, dest);
+}
+
+int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5) {
+    if (!ctx || !out5) return HZK_ERR_INVALID;
+    for (int i = 0; i < 5; ++i) out5[i] = ctx->last_msm[i];
+    return HZK_OK;
 }
 
 int hzk_timing_enable(hzk_ctx* ctx, int enable) {

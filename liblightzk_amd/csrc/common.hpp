@@ -10,6 +10,7 @@
 
 struct hzk_ctx {
     int device = 0;
+    size_t lds_per_block = 65536;   // device limit, read at creation
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string last_error;
@@ -27,6 +28,11 @@ struct hzk_ctx {
     std::vector<hipEvent_t> event_pool;
     // fault injection for tests of the error paths (host-side only)
     long fail_alloc_after = -1;
+    // fixed-base comb tables (device, built on first use)
+    void* comb_g1 = nullptr;
+    void* comb_g2 = nullptr;
+    // last MSM: entries, segments, largest bucket, window bits, windows
+    uint32_t last_msm[5] = {0, 0, 0, 0, 0};
     // autotune overrides (0 = automatic)
     int force_window = 0;
     int force_seg = 0;

@@ -8,6 +8,7 @@
 #pragma once
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 namespace hzk {
 namespace host {
@@ -188,6 +189,46 @@ struct Xyzz {
 
 typedef Xyzz<Fq> G1x;
 typedef Xyzz<Fq2> G2x;
+
+// Generators (libff alt_bn128_init.cu:130-132, :191-195), Montgomery form.
+inline G1x g1_generator() {
+    Fq x = Fq::one();
+    Fq y = x.dbl();
+    return G1x::from_affine(x, y);
+}
+inline Fq fq_from_canonical(const uint64_t c[4]) { Fq t; std::memcpy(t.v, c, 32); Fq r2; std::memcpy(r2.v, FQ.r2, 32); return t * r2; }
+inline G2x g2_generator() {
+    static const uint64_t xc0[4] = {0x46debd5cd992f6edULL, 0x674322d4f75edaddULL, 0x426a00665e5c4479ULL, 0x1800deef121f1e76ULL};
+    static const uint64_t xc1[4] = {0x97e485b7aef312c2ULL, 0xf1aa493335a9e712ULL, 0x7260bfb731fb5d25ULL, 0x198e9393920d483aULL};
+    static const uint64_t yc0[4] = {0x4ce6cc0166fa7daaULL, 0xe3d1e7690c43d37bULL, 0x4aab71808dcb408fULL, 0x12c85ea5db8c6debULL};
+    static const uint64_t yc1[4] = {0x55acdadcd122975bULL, 0xbc4b313370b38ef3ULL, 0xec9e99ad690c3395ULL, 0x090689d0585ff075ULL};
+    Fq2 x{fq_from_canonical(xc0), fq_from_canonical(xc1)};
+    Fq2 y{fq_from_canonical(yc0), fq_from_canonical(yc1)};
+    return G2x::from_affine(x, y);
+}
+
+// Montgomery's trick: affine (x, y) of every point with one inversion.
+template <class F>
+void batch_affine(const std::vector<Xyzz<F>>& pts, std::vector<F>& xs, std::vector<F>& ys) {
+    size_t n = pts.size();
+    xs.assign(n, F::zero());
+    ys.assign(n, F::zero());
+    std::vector<F> pre(n);
+    F acc = F::one();
+    for (size_t i = 0; i < n; ++i) {
+        pre[i] = acc;
+        if (!pts[i].is_inf()) acc = acc * (pts[i].ZZ * pts[i].ZZZ);
+    }
+    F inv = acc.inverse();
+    for (size_t i = n; i-- > 0;) {
+        if (pts[i].is_inf()) continue;
+        F d = pts[i].ZZ * pts[i].ZZZ;
+        F di = inv * pre[i];   // 1 / (ZZ ZZZ)
+        inv = inv * d;
+        xs[i] = pts[i].X * (di * pts[i].ZZZ);
+        ys[i] = pts[i].Y * (di * pts[i].ZZ);
+    }
+}
 
 }  // namespace host
 }  // namespace hzk

@@ -7,25 +7,29 @@
 //
 //  1. recode   one lane per scalar: leave Montgomery form, signed c-bit digits
 //              d_j in [-2^(c-1), 2^(c-1)] over W = ceil(255/c) windows (the
-//              top window cannot carry because scalars are < r < 2^254), and a
-//              histogram of (window, |d|) keys; digit 0 is dropped (the
-//              reference accumulates bucket 0 and scales it by 0, msm.cuh:121-161).
-//              Lanes of a wave that hit the same key (the 70%-{0,1} witness case
-//              puts a third of all points in one bucket) are merged into one
-//              atomic by a leader ballot.
-//  2. scan     device-wide exclusive scan of (count, ceil(count/K)) per key ->
-//              bucket offsets and level-1 segment bases; totals and the largest
-//              bucket go back to the host in one small copy.
-//  3. scatter  lane per scalar writes point index | sign<<31 into its bucket.
-//  4. accumulate  lane per segment of <= K entries of one bucket: mixed
+//              top window cannot carry because scalars are < r < 2^254),
+//              written window-major; digit 0 is dropped (the reference
+//              accumulates bucket 0 and scales it by 0, msm.cuh:121-161).
+//  2. histogram  one workgroup per (tile of scalars, window): counts in LDS
+//              (2^(c-1) counters, <= 128 KiB), one coalesced row per tile. No
+//              device-scope atomics: on a multi-XCD part they resolve past the
+//              per-XCD L2 and ran at ~5 G/s, slower than the whole accumulation.
+//  3. scan     per key a prefix over tiles, then a device-wide exclusive scan
+//              of (count, ceil(count/K)) -> bucket offsets and level-1 segment
+//              bases; totals and the largest bucket go back to the host in one
+//              small copy.
+//  4. scatter  same (tile, window) workgroups: LDS cursors seeded from the
+//              scan, LDS atomics hand out slots, point index | sign<<31 is
+//              written into its bucket.
+//  5. accumulate  lane per segment of <= K entries of one bucket: mixed
 //              additions of affine bases into an XYZZ accumulator. Buckets of any
 //              size are split into segments, so a hot bucket never serialises.
-//  5. merge    strided in-place tree over each bucket's segment sums (K2-ary),
+//  6. merge    strided in-place tree over each bucket's segment sums (K2-ary),
 //              only as many levels as the largest bucket needs.
-//  6. bucket reduction  lane per run of g buckets: running sums give
+//  7. bucket reduction  lane per run of g buckets: running sums give
 //              sum (d - lo) B_d and sum B_d; the segment's weight lo is applied
 //              by double-and-add; a block per window then tree-sums segments.
-//  7. window combine on the host (Horner with c doublings per window) and
+//  8. window combine on the host (Horner with c doublings per window) and
 //              normalisation; see host_ec.hpp for why this tail runs on the CPU.
 #include <algorithm>
 #include <cstring>
@@ -35,7 +39,7 @@
 
 namespace hzk {
 
-static constexpr uint32_t KEY_NONE = 0xffffffffu;
+static constexpr uint32_t DIGIT_NEG = 0x80000000u;
 
 // ------------------------------------------------------------------ digits
 template <int C>
@@ -48,81 +52,76 @@ HZK_DEV uint32_t window_bits(const fr_t& s, int j) {
     return (uint32_t)(v >> sh) & MASK;
 }
 
-// Signed digit of window j given the incoming carry; returns |d| (0 = none).
+// digits[j * n + i] = |d_ij| | sign << 31, or 0 for a zero digit
 template <int C>
-HZK_DEV uint32_t signed_digit(const fr_t& s, int j, uint32_t& carry, bool& neg) {
-    constexpr uint32_t NB = 1u << (C - 1);
-    uint32_t raw = window_bits<C>(s, j) + carry;
-    if (raw > NB) { carry = 1; neg = true; return 2 * NB - raw; }
-    carry = 0; neg = false; return raw;
-}
-
-HZK_DEV uint32_t lane_id() { return __lane_id(); }
-
-// counter[key] += 1 for every lane with key != KEY_NONE; lanes sharing the
-// first active key are merged into one atomic.
-HZK_DEV void wave_count(uint32_t* counter, uint32_t key) {
-    uint64_t valid = __ballot(key != KEY_NONE);
-    if (valid == 0) return;
-    int leader = __ffsll((unsigned long long)valid) - 1;
-    uint32_t lkey = __shfl(key, leader);
-    uint64_t match = __ballot(key == lkey);
-    if ((int)lane_id() == leader) atomicAdd(&counter[lkey], (uint32_t)__popcll(match));
-    else if (key != KEY_NONE && key != lkey) atomicAdd(&counter[key], 1u);
-}
-// pos = atomic slot in cursor[key]; same leader merge as wave_count.
-HZK_DEV uint32_t wave_slot(uint32_t* cursor, uint32_t key) {
-    uint64_t valid = __ballot(key != KEY_NONE);
-    if (valid == 0) return 0;
-    int leader = __ffsll((unsigned long long)valid) - 1;
-    uint32_t lkey = __shfl(key, leader);
-    uint64_t match = __ballot(key == lkey);
-    uint32_t base = 0;
-    if ((int)lane_id() == leader) base = atomicAdd(&cursor[lkey], (uint32_t)__popcll(match));
-    base = __shfl(base, leader);
-    uint32_t pos = 0;
-    if (key == lkey) pos = base + (uint32_t)__popcll(match & ((1ull << lane_id()) - 1));
-    else if (key != KEY_NONE) pos = atomicAdd(&cursor[key], 1u);
-    return pos;
-}
-
-template <int C>
-__global__ void __launch_bounds__(256) k_msm_count(const fr_t* __restrict__ scalars, uint32_t n,
-                                                  fr_t* __restrict__ canon, uint32_t* __restrict__ count) {
+__global__ void __launch_bounds__(256) k_msm_digits(const fr_t* __restrict__ scalars, uint32_t n,
+                                                   uint32_t* __restrict__ digits) {
     constexpr int W = (255 + C - 1) / C;
     constexpr uint32_t NB = 1u << (C - 1);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < n;
-    fr_t s = fr_t::zero();
-    if (active) {
-        s = scalars[i].from_mont();
-        canon[i] = s;
-    }
+    if (i >= n) return;
+    fr_t s = scalars[i].from_mont();
     uint32_t carry = 0;
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        bool neg;
-        uint32_t d = signed_digit<C>(s, j, carry, neg);
-        wave_count(count, (active && d) ? j * NB + d - 1 : KEY_NONE);
+        uint32_t raw = window_bits<C>(s, j) + carry;
+        uint32_t d;
+        if (raw > NB) { carry = 1; d = (2 * NB - raw) | DIGIT_NEG; }
+        else { carry = 0; d = raw; }
+        if ((d & ~DIGIT_NEG) == 0) d = 0;
+        digits[(size_t)j * n + i] = d;
     }
 }
 
-template <int C>
-__global__ void __launch_bounds__(256) k_msm_scatter(const fr_t* __restrict__ canon, uint32_t n,
-                                                    uint32_t* __restrict__ cursor, uint32_t* __restrict__ sorted) {
-    constexpr int W = (255 + C - 1) / C;
-    constexpr uint32_t NB = 1u << (C - 1);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < n;
-    fr_t s = active ? canon[i] : fr_t::zero();
-    uint32_t carry = 0;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        bool neg;
-        uint32_t d = signed_digit<C>(s, j, carry, neg);
-        uint32_t key = (active && d) ? j * NB + d - 1 : KEY_NONE;
-        uint32_t pos = wave_slot(cursor, key);
-        if (key != KEY_NONE) sorted[pos] = i | (neg ? 0x80000000u : 0u);
+// tile t of window j covers scalars [t*tile, min(n, (t+1)*tile))
+__global__ void __launch_bounds__(1024) k_msm_hist(const uint32_t* __restrict__ digits, uint32_t n, uint32_t tile,
+                                                   uint32_t NB, uint32_t nk, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t cnt[];
+    const uint32_t t = blockIdx.x, j = blockIdx.y;
+    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) cnt[b] = 0;
+    __syncthreads();
+    const uint32_t i0 = t * tile, i1 = min(n, i0 + tile);
+    const uint32_t* dj = digits + (size_t)j * n;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        uint32_t d = dj[i] & ~DIGIT_NEG;
+        if (d) atomicAdd(&cnt[d - 1], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = hist + (size_t)t * nk + (size_t)j * NB;
+    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) row[b] = cnt[b];
+}
+
+// per key: count = sum over tiles, hist[t][key] <- exclusive prefix over tiles
+__global__ void __launch_bounds__(256) k_msm_colscan(uint32_t* __restrict__ hist, uint32_t nk, uint32_t T,
+                                                     uint32_t* __restrict__ count) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    uint32_t run = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        uint32_t v = hist[(size_t)t * nk + k];
+        hist[(size_t)t * nk + k] = run;
+        run += v;
+    }
+    count[k] = run;
+}
+
+__global__ void __launch_bounds__(1024) k_msm_scatter(const uint32_t* __restrict__ digits, uint32_t n, uint32_t tile,
+                                                      uint32_t NB, uint32_t nk, const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ offs, uint32_t* __restrict__ sorted) {
+    extern __shared__ uint32_t cur[];
+    const uint32_t t = blockIdx.x, j = blockIdx.y;
+    const uint32_t* row = hist + (size_t)t * nk + (size_t)j * NB;
+    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) cur[b] = offs[j * NB + b] + row[b];
+    __syncthreads();
+    const uint32_t i0 = t * tile, i1 = min(n, i0 + tile);
+    const uint32_t* dj = digits + (size_t)j * n;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        uint32_t d = dj[i];
+        uint32_t mag = d & ~DIGIT_NEG;
+        if (mag) {
+            uint32_t pos = atomicAdd(&cur[mag - 1], 1u);
+            sorted[pos] = i | (d & DIGIT_NEG);
+        }
     }
 }
 
@@ -137,6 +136,7 @@ HZK_DEV uint64_t pack_count(uint32_t c, uint32_t K) { return ((uint64_t)((c + K 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __restrict__ count, uint32_t nk, uint32_t K,
                                                              uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ stats) {
     __shared__ uint64_t red[SCAN_THREADS];
+    __shared__ uint32_t redmax[SCAN_THREADS];
     uint32_t base = blockIdx.x * SCAN_TILE;
     uint64_t s = 0;
     uint32_t mx = 0;
@@ -145,40 +145,54 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __r
         if (k < nk) { uint32_t c = count[k]; s += pack_count(c, K); mx = max(mx, c); }
     }
     red[threadIdx.x] = s;
+    redmax[threadIdx.x] = mx;
     __syncthreads();
     for (int off = SCAN_THREADS / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) {
+            red[threadIdx.x] += red[threadIdx.x + off];
+            redmax[threadIdx.x] = max(redmax[threadIdx.x], redmax[threadIdx.x + off]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = red[0];
-    if (mx) atomicMax(&stats[2], mx);
+    if (threadIdx.x == 0) {
+        tile_sum[blockIdx.x] = red[0];
+        tile_sum[gridDim.x + blockIdx.x] = redmax[0];
+    }
 }
 
-// single block: exclusive scan of the tile sums in place, total to stats[0..1]
+// single block: exclusive scan of the tile sums in place; totals and max to stats
 __global__ void __launch_bounds__(1024) k_scan_top(uint64_t* __restrict__ tile_sum, uint32_t ntiles, uint32_t* __restrict__ stats) {
     __shared__ uint64_t part[1024];
+    __shared__ uint32_t pmax[1024];
     uint32_t per = (ntiles + 1023) / 1024;
     uint32_t b = threadIdx.x * per, e = min(ntiles, b + per);
     uint64_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += tile_sum[i];
+    uint32_t mx = 0;
+    for (uint32_t i = b; i < e; ++i) { s += tile_sum[i]; mx = max(mx, (uint32_t)tile_sum[ntiles + i]); }
     part[threadIdx.x] = s;
+    pmax[threadIdx.x] = mx;
     __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partials
     for (int off = 1; off < 1024; off <<= 1) {
         uint64_t v = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        uint32_t m = (int)threadIdx.x >= off ? pmax[threadIdx.x - off] : 0;
         __syncthreads();
         part[threadIdx.x] += v;
+        pmax[threadIdx.x] = max(pmax[threadIdx.x], m);
         __syncthreads();
     }
     uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
     for (uint32_t i = b; i < e; ++i) { uint64_t v = tile_sum[i]; tile_sum[i] = run; run += v; }
-    if (threadIdx.x == 1023) { stats[0] = (uint32_t)part[1023]; stats[1] = (uint32_t)(part[1023] >> 32); }
+    if (threadIdx.x == 1023) {
+        stats[0] = (uint32_t)part[1023];
+        stats[1] = (uint32_t)(part[1023] >> 32);
+        stats[2] = pmax[1023];
+    }
 }
 
-// per tile: exclusive scan, write offsets / segment bases / cursors
+// per tile: exclusive scan, write bucket offsets and segment bases
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __restrict__ count, uint32_t nk, uint32_t K,
                                                              const uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ offs,
-                                                             uint32_t* __restrict__ segbase, uint32_t* __restrict__ cursor) {
+                                                             uint32_t* __restrict__ segbase) {
     __shared__ uint64_t part[SCAN_THREADS];
     uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint64_t v[SCAN_ITEMS];
@@ -202,83 +216,111 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
         if (k <= nk) {
             offs[k] = (uint32_t)run;
             segbase[k] = (uint32_t)(run >> 32);
-            if (k < nk) cursor[k] = (uint32_t)run;
         }
         run += v[t];
     }
 }
 
-// ------------------------------------------------------------------ driver
+// ------------------------------------------------------------------ driver pieces
 template <int C>
-static void launch_recode(hzk_ctx* ctx, const fr_t* scalars, uint32_t n, fr_t* canon, uint32_t* count) {
-    TimedLaunch tl(ctx, "msm_count");
-    hipLaunchKernelGGL(k_msm_count<C>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, scalars, n, canon, count);
+static void launch_digits(hzk_ctx* ctx, const fr_t* scalars, uint32_t n, uint32_t* digits) {
+    TimedLaunch tl(ctx, "msm_digits");
+    hipLaunchKernelGGL(k_msm_digits<C>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, scalars, n, digits);
 }
-template <int C>
-static void launch_scatter(hzk_ctx* ctx, const fr_t* canon, uint32_t n, uint32_t* cursor, uint32_t* sorted) {
-    TimedLaunch tl(ctx, "msm_scatter");
-    hipLaunchKernelGGL(k_msm_scatter<C>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, canon, n, cursor, sorted);
-}
-
-typedef void (*recode_fn)(hzk_ctx*, const fr_t*, uint32_t, fr_t*, uint32_t*);
-typedef void (*scatter_fn)(hzk_ctx*, const fr_t*, uint32_t, uint32_t*, uint32_t*);
+typedef void (*digits_fn)(hzk_ctx*, const fr_t*, uint32_t, uint32_t*);
 template <int C>
 struct CTab {
-    static void fill(recode_fn* r, scatter_fn* s) {
-        r[C] = launch_recode<C>;
-        s[C] = launch_scatter<C>;
-        CTab<C - 1>::fill(r, s);
+    static void fill(digits_fn* d) {
+        d[C] = launch_digits<C>;
+        CTab<C - 1>::fill(d);
     }
 };
 template <>
 struct CTab<MSM_MIN_C - 1> {
-    static void fill(recode_fn*, scatter_fn*) {}
+    static void fill(digits_fn*) {}
 };
+struct StageTables {
+    digits_fn digits[MSM_MAX_C + 1];
+    StageTables() { CTab<MSM_MAX_C>::fill(digits); }
+};
+static const StageTables& stage_tables() {
+    static StageTables t;
+    return t;
+}
 
-MsmPlan msm_plan(size_t n, int force_c, int force_g) {
+MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     MsmPlan p;
     int lg = 0;
     while (((size_t)1 << (lg + 1)) <= n) ++lg;
     int c = lg - 8;
     if (c < 8) c = 8;
     if (c > 16) c = 16;
+    while (c > MSM_MIN_C && ((size_t)4 << (c - 1)) > lds_bytes) --c;   // histogram must fit in LDS
     if (force_c) c = force_c;
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.NB = 1u << (c - 1);
-    p.K = 64;
+    // segment length: long enough to amortise the segment bookkeeping, short
+    // enough that about 2^17 lanes share the work (per-lane point additions
+    // are latency-bound below that)
+    size_t L = n * (size_t)p.W;
+    uint32_t K = 64;
+    while (K > 8 && L / K < ((size_t)1 << 17)) K >>= 1;
+    p.K = K;
     uint32_t g = p.NB / 512;
     if (g < 4) g = 4;
     if (g > 16) g = 16;
     if (g > p.NB) g = p.NB;
     if (force_g) g = force_g;
     p.g = g;
+    // scalar tiles per window for the LDS histogram / scatter workgroups
+    uint32_t T = (512 + p.W - 1) / p.W;
+    while (T > 1 && n / T < 4096) T >>= 1;
+    p.T = T;
+    p.tile = (uint32_t)((n + T - 1) / T);
     return p;
 }
 
-struct StageTables {
-    recode_fn recode[MSM_MAX_C + 1];
-    scatter_fn scatter[MSM_MAX_C + 1];
-    StageTables() { CTab<MSM_MAX_C>::fill(recode, scatter); }
-};
-static const StageTables& stage_tables() {
-    static StageTables t;
-    return t;
+// Allow the histogram / scatter workgroups the LDS the largest window needs.
+static void set_lds_limits() {
+    static bool done = false;
+    if (done) return;
+    done = true;
+    int bytes = (1 << (MSM_MAX_C - 1)) * 4;
+    (void)hipFuncSetAttribute((const void*)k_msm_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_msm_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
-void msm_launch_recode(hzk_ctx* ctx, int c, const void* scalars, uint32_t n, void* canon, uint32_t* count) {
-    stage_tables().recode[c](ctx, (const fr_t*)scalars, n, (fr_t*)canon, count);
-}
-void msm_launch_scatter(hzk_ctx* ctx, int c, const void* canon, uint32_t n, uint32_t* cursor, uint32_t* sorted) {
-    stage_tables().scatter[c](ctx, (const fr_t*)canon, n, cursor, sorted);
-}
-uint32_t msm_scan_tiles(uint32_t nk) { return (nk + SCAN_TILE) / SCAN_TILE; }   // covers index nk (the total)
-void msm_launch_scan(hzk_ctx* ctx, const uint32_t* count, uint32_t nk, uint32_t K, uint64_t* tiles, uint32_t ntiles,
-                     uint32_t* stats, uint32_t* offs, uint32_t* segbase, uint32_t* cursor) {
-    TimedLaunch tl(ctx, "msm_scan");
+
+size_t msm_lds_bytes(const MsmPlan& p) { return (size_t)p.NB * 4; }
+
+void msm_launch_sort(hzk_ctx* ctx, const MsmPlan& p, const void* scalars, uint32_t n, uint32_t* digits, uint32_t* hist,
+                     uint32_t* count, uint64_t* tiles, uint32_t ntiles, uint32_t* stats, uint32_t* offs,
+                     uint32_t* segbase) {
+    const uint32_t nk = (uint32_t)p.W * p.NB;
     hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, K, tiles, stats);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, ntiles, stats);
-    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, K, tiles, offs, segbase, cursor);
+    set_lds_limits();
+    stage_tables().digits[p.c](ctx, (const fr_t*)scalars, n, digits);
+    {
+        TimedLaunch tl(ctx, "msm_hist");
+        hipLaunchKernelGGL(k_msm_hist, dim3(p.T, p.W), dim3(1024), p.NB * 4, st, digits, n, p.tile, p.NB, nk, hist);
+    }
+    {
+        TimedLaunch tl(ctx, "msm_scan");
+        hipLaunchKernelGGL(k_msm_colscan, dim3(blocks_for(nk, 256)), dim3(256), 0, st, hist, nk, p.T, count);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, p.K, tiles, stats);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, ntiles, stats);
+        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, p.K, tiles, offs, segbase);
+    }
 }
+
+void msm_launch_scatter(hzk_ctx* ctx, const MsmPlan& p, const uint32_t* digits, uint32_t n, const uint32_t* hist,
+                        const uint32_t* offs, uint32_t* sorted) {
+    const uint32_t nk = (uint32_t)p.W * p.NB;
+    TimedLaunch tl(ctx, "msm_scatter");
+    hipLaunchKernelGGL(k_msm_scatter, dim3(p.T, p.W), dim3(1024), p.NB * 4, ctx->stream, digits, n, p.tile, p.NB, nk, hist,
+                       offs, sorted);
+}
+
+uint32_t msm_scan_tiles(uint32_t nk) { return (nk + SCAN_TILE) / SCAN_TILE; }   // covers index nk (the total)
 
 }  // namespace hzk

@@ -131,7 +131,9 @@ int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size
     HZK_CHECK(ctx, scalars && bases && dest, HZK_ERR_INVALID, "msm: null pointer");
     HZK_CHECK(ctx, win_siz == 0 || (win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C), HZK_ERR_INVALID,
               "msm: win_siz=%zu unsupported (0 or %d..%d)", win_siz, MSM_MIN_C, MSM_MAX_C);
-    MsmPlan p = msm_plan(n, win_siz ? (int)win_siz : ctx->force_window, ctx->force_seg);
+    MsmPlan p = msm_plan(n, win_siz ? (int)win_siz : ctx->force_window, ctx->force_seg, ctx->lds_per_block);
+    HZK_CHECK(ctx, msm_lds_bytes(p) <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
+              "msm: window %d needs %zu B of LDS per workgroup, device allows %zu", p.c, msm_lds_bytes(p), ctx->lds_per_block);
     HZK_CHECK(ctx, p.g >= 1 && (p.g & (p.g - 1)) == 0 && p.g <= p.NB, HZK_ERR_INVALID, "msm: bad segment size %u", p.g);
     const uint32_t nk = (uint32_t)p.W * p.NB;
     const size_t Lmax = n * (size_t)p.W;
@@ -141,12 +143,12 @@ int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size
 
     size_t need = 0;
     auto add = [&need](size_t b) { need = ((need + 255) & ~size_t(255)) + b; };
-    add(n * sizeof(fr_t));                      // canon
+    add(Lmax * 4ull);                           // digits (window-major)
+    add((size_t)p.T * nk * 4ull);               // per-tile histograms
     add(nk * 4ull);                             // count
     add((nk + 1) * 4ull);                       // offs
     add((nk + 1) * 4ull);                       // segbase
-    add(nk * 4ull);                             // cursor
-    add(ntiles * 8ull);                         // tile sums
+    add(2ull * ntiles * 8ull);                  // scan tile sums + maxima
     add(64);                                    // stats
     add(Lmax * 4ull);                           // sorted
     add(Smax * 4ull);                           // seg2key
@@ -158,12 +160,12 @@ int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size
     rc = ensure_pinned(ctx, 64 + p.W * sizeof(xyzz_t<F>));
     if (rc) return rc;
     Arena ar{(char*)ctx->scratch, ctx->scratch_bytes};
-    fr_t* canon = (fr_t*)ar.take(n * sizeof(fr_t));
+    uint32_t* digits = (uint32_t*)ar.take(Lmax * 4ull);
+    uint32_t* hist = (uint32_t*)ar.take((size_t)p.T * nk * 4ull);
     uint32_t* count = (uint32_t*)ar.take(nk * 4ull);
     uint32_t* offs = (uint32_t*)ar.take((nk + 1) * 4ull);
     uint32_t* segbase = (uint32_t*)ar.take((nk + 1) * 4ull);
-    uint32_t* cursor = (uint32_t*)ar.take(nk * 4ull);
-    uint64_t* tiles = (uint64_t*)ar.take(ntiles * 8ull);
+    uint64_t* tiles = (uint64_t*)ar.take(2ull * ntiles * 8ull);
     uint32_t* stats = (uint32_t*)ar.take(64);
     uint32_t* sorted = (uint32_t*)ar.take(Lmax * 4ull);
     uint32_t* seg2key = (uint32_t*)ar.take(Smax * 4ull);
@@ -172,17 +174,17 @@ int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size
     xyzz_t<F>* winsum = (xyzz_t<F>*)ar.take((size_t)p.W * sizeof(xyzz_t<F>));
 
     hipStream_t st = ctx->stream;
-    HZK_HIP(ctx, hipMemsetAsync(count, 0, nk * 4ull, st));
-    HZK_HIP(ctx, hipMemsetAsync(stats, 0, 64, st));
-    msm_launch_recode(ctx, p.c, scalars, (uint32_t)n, canon, count);
-    msm_launch_scan(ctx, count, nk, p.K, tiles, ntiles, stats, offs, segbase, cursor);
+    msm_launch_sort(ctx, p, scalars, (uint32_t)n, digits, hist, count, tiles, ntiles, stats, offs, segbase);
     HZK_HIP(ctx, hipGetLastError());
     uint32_t* hstats = (uint32_t*)ctx->pinned;
     HZK_HIP(ctx, hipMemcpyAsync(hstats, stats, 16, hipMemcpyDeviceToHost, st));
-    msm_launch_scatter(ctx, p.c, canon, (uint32_t)n, cursor, sorted);
+    msm_launch_scatter(ctx, p, digits, (uint32_t)n, hist, offs, sorted);
+    HZK_HIP(ctx, hipGetLastError());
     HZK_HIP(ctx, hipStreamSynchronize(st));
     const uint32_t L = hstats[0], S = hstats[1], maxc = hstats[2];
     HZK_CHECK(ctx, L <= Lmax && S <= Smax, HZK_ERR_INVALID, "msm: internal size check failed (L=%u S=%u)", L, S);
+    ctx->last_msm[0] = L; ctx->last_msm[1] = S; ctx->last_msm[2] = maxc;
+    ctx->last_msm[3] = (uint32_t)p.c; ctx->last_msm[4] = (uint32_t)p.W;
 
     if (S > 0) {
         {

@@ -36,3 +36,13 @@ def test_context_creation_fails_cleanly_without_gpu():
                os.path.dirname(os.path.abspath(liblightzk_amd.__file__)))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert out.stdout.strip() in ("gpu", "nogpu"), out.stderr
+
+
+def test_product_generator_matches_oracle(orc):
+    """The product's seeded generator and the oracle's produce identical bytes."""
+    import numpy as np
+    for kind, okind in (("fr", False), ("fr_sparse", True)):
+        a = liblightzk_amd.gen_field(5, 3, 5000, kind)
+        b = orc.gen_fr(5, 3, 5000, sparse=okind)
+        assert np.array_equal(a, b), kind
+    assert np.array_equal(liblightzk_amd.gen_field(6, 1, 999, "fq"), orc.gen_fq(6, 1, 999))
