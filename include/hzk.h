@@ -76,6 +76,38 @@ int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
  * digits), accumulation segments, largest bucket, window bits c, windows} */
 int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5);
 
+/* ---- NTT over Fr (replaces number_theory_transformation,
+ * include/operator/poly.cuh:65-73, same semantics): in place, natural order,
+ *   coeffs[k] <- factor * sum_j coeffs[j] * w^(j k),  deg a power of two <= 2^26.
+ * omega_powers: device table w^e for e < deg/2 (ntt_setup layout,
+ * include/protocol/groth16-operator.cuh:6-32), or NULL for the library's own
+ * table of the primitive deg-th root (inverse = 0) or its inverse (inverse = 1,
+ * then scaled by 1/deg unless factor is given). factor: device Fr or NULL. */
+int hzk_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, const void* omega_powers, int inverse, const void* factor);
+/* coset NTT on g*<w>, g = 5 (alt_bn128 Fr multiplicative generator,
+ * alt_bn128_init.cu:50): inverse = 0 evaluates the coefficients at g*w^k;
+ * inverse = 1 interpolates from those evaluations back to coefficients.
+ * No reference counterpart (the reference evaluates on a 2*deg domain
+ * instead, include/protocol/groth16.cuh:176-185). */
+int hzk_coset_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, int inverse);
+
+/* ---- R1CS SpMV (replaces batch_sparse_matrix_vector_multiplication,
+ * include/operator/poly.cuh:5-30; CSR = the reference's spmat_descript,
+ * include/data-structure/xpu-spmat.cuh:20-25: device arrays, row_ptr has `row`
+ * entries and no sentinel, the last row ends at cnt). Az/Bz/Cz need not be
+ * zeroed (the reference requires it, poly.cuh:8). */
+typedef struct hzk_csr {
+    const void* val;            /* Fr[cnt] */
+    const uint64_t* col_idx;    /* [cnt] */
+    const uint64_t* row_ptr;    /* [row] */
+    size_t row, col, cnt;
+} hzk_csr;
+int hzk_spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, const void* z, void* Az, void* Bz,
+              void* Cz);
+/* A <- A o B - C over deg Fr elements (replaces polynomial_A_times_B_minus_C,
+ * include/operator/poly.cuh:75-86) */
+int hzk_pointwise_abc(hzk_ctx* ctx, size_t deg, void* A, const void* B, const void* C);
+
 /* ---- fixed-base generation (replaces fix_base_multi_scalar_multiplication_g1/_g2,
  * include/operator/msm.cuh:5-46, plus the affine normalisation of pre_comp_g1/_g2,
  * msm.cuh:48-75): out_affine[i] = k_i * G for device Fr scalars k (Montgomery);
@@ -88,6 +120,26 @@ int hzk_fixbase_g2(hzk_ctx* ctx, const void* scalars, size_t n, void* out_affine
  * include/host-arith/field/fp.cuh:789-816). kind: 0 uniform Fr, 1 sparse Fr
  * (70% in {0,1}), 2 uniform Fq; n elements of 32 B into host memory. */
 int hzk_gen_field(uint64_t seed, uint64_t stream, size_t n, int kind, void* host_out);
+
+/* ---- Groth16 prover (replaces the protocol class Groth16<...>,
+ * include/protocol/groth16.cuh:18-223: the constructor + Setup(win_siz) become
+ * hzk_g16_setup, Prove() becomes hzk_g16_prove, which returns the proof instead
+ * of printing it, groth16.cuh:219-221). The statement is the synthetic
+ * multiplication-gate circuit with M = 2^log_m constraint rows (circuit and
+ * toxic waste from `seed`); the proving key stays resident in HBM. */
+typedef struct hzk_g16 hzk_g16;
+int hzk_g16_setup(hzk_ctx* ctx, uint64_t seed, int log_m, int num_public, hzk_g16** out);
+void hzk_g16_free(hzk_g16* g);
+/* {M, num_vars, num_public, num_gates, nnz A, nnz B, nnz C, proving-key bytes} */
+int hzk_g16_info(const hzk_g16* g, uint64_t* out8);
+/* seeded satisfying witness z[num_vars] (host, Fr Montgomery; z[0] = 1) */
+int hzk_g16_witness(const hzk_g16* g, uint64_t wseed, void* z_host);
+/* prover randomness r | s (host, 2 x 32 B) for a witness seed */
+int hzk_g16_rs(uint64_t wseed, void* rs_host);
+/* proof = A (G1 affine 64 B) | B (G2 affine 128 B) | C (G1 affine 64 B), host memory */
+int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_host, void* proof_host);
+/* device pointer / count of a key array: 0 A1, 1 B1, 2 B2 (G2), 3 L1, 4 H1 */
+int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count);
 
 /* ---- timing (replaces the reference's Timer phase printouts,
  * include/timer/timer.cuh:5-12). When enabled, each kernel launch is

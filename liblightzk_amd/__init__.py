@@ -171,6 +171,13 @@ class Context:
         self._check(load().hzk_field_op(self.handle, fields[field], ops[op], int(a), int(b) if b is not None else None,
                                         int(out), n))
 
+    def ntt(self, coeffs, deg, omega_powers=None, inverse=False, factor=None):
+        self._check(load().hzk_ntt(self.handle, int(coeffs), deg, int(omega_powers) if omega_powers is not None else None,
+                                   1 if inverse else 0, int(factor) if factor is not None else None))
+
+    def coset_ntt(self, coeffs, deg, inverse=False):
+        self._check(load().hzk_coset_ntt(self.handle, int(coeffs), deg, 1 if inverse else 0))
+
     def msm_last_stats(self):
         a = (ctypes.c_uint32 * 5)()
         self._check(load().hzk_msm_last_stats(self.handle, a))
@@ -189,6 +196,74 @@ class Context:
         self._check(load().hzk_msm_g2(self.handle, int(scalars), int(bases), n, 0, win_siz, int(dest)))
 
 
+class Csr(ctypes.Structure):
+    """hzk_csr: the reference's spmat_descript (xpu-spmat.cuh:20-25) over device arrays."""
+    _fields_ = [("val", ctypes.c_void_p), ("col_idx", ctypes.c_void_p), ("row_ptr", ctypes.c_void_p),
+                ("row", ctypes.c_size_t), ("col", ctypes.c_size_t), ("cnt", ctypes.c_size_t)]
+
+
+def batch_sparse_matrix_vector_multiplication(ctx, A, B, C, z, Az, Bz, Cz):
+    """Reference operator poly.cuh:5-30: Az, Bz, Cz <- A z, B z, C z (Csr descriptors, device vectors)."""
+    ctx._check(load().hzk_spmv3(ctx.handle, ctypes.byref(A), ctypes.byref(B), ctypes.byref(C), int(z), int(Az),
+                                int(Bz), int(Cz)))
+
+
+def polynomial_A_times_B_minus_C(ctx, deg, A, B, C):
+    """Reference operator poly.cuh:75-86: A <- A o B - C."""
+    ctx._check(load().hzk_pointwise_abc(ctx.handle, deg, int(A), int(B), int(C)))
+
+
+class Groth16:
+    """Reference protocol class (include/protocol/groth16.cuh:18-223) on the GPU.
+
+    Groth16(ctx, seed, log_m) builds the synthetic circuit and runs Setup (keys
+    resident in HBM); prove(z, rs) returns the 256-byte proof A | B | C."""
+
+    def __init__(self, ctx, seed, log_m, num_public=8):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._check(load().hzk_g16_setup(ctx.handle, seed, log_m, num_public, ctypes.byref(h)))
+        self.h = h.value
+        info = (ctypes.c_uint64 * 8)()
+        load().hzk_g16_info(self.h, info)
+        keys = ["M", "num_vars", "num_public", "num_gates", "nnzA", "nnzB", "nnzC", "key_bytes"]
+        self.info = {k: int(v) for k, v in zip(keys, info)}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().hzk_g16_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def witness(self, wseed):
+        z = np.empty(4 * self.info["num_vars"], dtype=np.uint64)
+        load().hzk_g16_witness(self.h, wseed, z.ctypes.data)
+        return z
+
+    @staticmethod
+    def rs(wseed):
+        out = np.empty(8, dtype=np.uint64)
+        load().hzk_g16_rs(wseed, out.ctypes.data)
+        return out
+
+    def prove(self, z, rs):
+        z = np.ascontiguousarray(z, dtype=np.uint64)
+        rs = np.ascontiguousarray(rs, dtype=np.uint64)
+        proof = np.empty(32, dtype=np.uint64)
+        self.ctx._check(load().hzk_g16_prove(self.ctx.handle, self.h, z.ctypes.data, rs.ctypes.data, proof.ctypes.data))
+        return proof
+
+    def key_array(self, which):
+        p, n = ctypes.c_void_p(), ctypes.c_size_t()
+        self.ctx._check(load().hzk_g16_key_array(self.h, which, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+
 def gen_field(seed, stream, n, kind="fr"):
     """Seeded synthetic field elements (host numpy uint64, 4 limbs each)."""
     out = np.empty(4 * n, dtype=np.uint64)
@@ -197,6 +272,13 @@ def gen_field(seed, stream, n, kind="fr"):
     if rc != 0:
         raise HzkError(f"hzk_gen_field rc={rc}")
     return out
+
+
+def number_theory_transformation(ctx, deg, log_deg, coeffs, omega_powers, factor=None):
+    """Reference operator poly.cuh:65-73: coeffs <- factor * DFT(coeffs) with w = omega_powers[1]."""
+    if deg != 1 << log_deg:
+        raise HzkError("deg must equal 2^log_deg")
+    ctx.ntt(coeffs, deg, omega_powers, False, factor)
 
 
 def multi_scalar_multiplication_g1(ctx, length, scalars, points, dest, win_siz=0, win_cnt=None, bucket_cnt=None,

@@ -50,10 +50,21 @@ def load():
         "hzk_field_op": (i32, [vp, i32, i32, vp, vp, vp, sz]),
         "hzk_msm_g1": (i32, [vp, vp, vp, sz, u32, sz, vp]),
         "hzk_msm_g2": (i32, [vp, vp, vp, sz, u32, sz, vp]),
+        "hzk_ntt": (i32, [vp, vp, sz, vp, i32, vp]),
+        "hzk_coset_ntt": (i32, [vp, vp, sz, i32]),
         "hzk_msm_last_stats": (i32, [vp, ctypes.POINTER(ctypes.c_uint32)]),
         "hzk_fixbase_g1": (i32, [vp, vp, sz, vp]),
         "hzk_fixbase_g2": (i32, [vp, vp, sz, vp]),
         "hzk_gen_field": (i32, [ctypes.c_uint64, ctypes.c_uint64, sz, i32, vp]),
+        "hzk_spmv3": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "hzk_pointwise_abc": (i32, [vp, sz, vp, vp, vp]),
+        "hzk_g16_setup": (i32, [vp, ctypes.c_uint64, i32, i32, ctypes.POINTER(vp)]),
+        "hzk_g16_free": (None, [vp]),
+        "hzk_g16_info": (i32, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "hzk_g16_witness": (i32, [vp, ctypes.c_uint64, vp]),
+        "hzk_g16_rs": (i32, [ctypes.c_uint64, vp]),
+        "hzk_g16_prove": (i32, [vp, vp, vp, vp, vp]),
+        "hzk_g16_key_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
         "hzk_timing_enable": (i32, [vp, i32]),
         "hzk_timing_reset": (i32, [vp]),
         "hzk_timing_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]),

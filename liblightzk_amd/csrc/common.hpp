@@ -33,6 +33,8 @@ struct hzk_ctx {
     void* comb_g2 = nullptr;
     // last MSM: entries, segments, largest bucket, window bits, windows
     uint32_t last_msm[5] = {0, 0, 0, 0, 0};
+    // NTT twiddle / coset tables for the last transform size (ntt.hip)
+    void* ntt_cache = nullptr;
     // autotune overrides (0 = automatic)
     int force_window = 0;
     int force_seg = 0;
@@ -68,6 +70,12 @@ struct Arena {
 // Make sure ctx->scratch holds at least `bytes`; returns 0 or an error code.
 int ensure_scratch(hzk_ctx* ctx, size_t bytes);
 int ensure_pinned(hzk_ctx* ctx, size_t bytes);
+void ntt_release(hzk_ctx* ctx);
+int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, const void* z, void* Az, void* Bz, void* Cz);
+int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale);
+// in-place standard NTT (inverse: w^-1 and 1/n) and coset NTT over Fr (ntt.hip)
+int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
+int coset_ntt_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 
 // Kernel timing helpers (no-ops unless ctx->timing).
 struct TimedLaunch {

@@ -1,0 +1,384 @@
+// Groth16 over alt_bn128 on one GPU: synthetic circuit, setup from seeded
+// toxic waste, and the prover.
+//
+// Keeps the reference's protocol shape (include/protocol/groth16.cuh:18-223:
+// construct, Setup, Prove with phases SpMV -> NTTs -> MSMs -> host assembly)
+// but proves a real statement, which the reference does not (random keys and
+// witness, no verifier: groth16.cuh:84-132):
+//   circuit  multiplication-gate DAG: gate g defines z_v = (A_g . z)(B_g . z),
+//            v = 65 + g, A/B rows hold 1-3 entries with columns uniform over
+//            earlier variables and Fr coefficients; the last num_public+1
+//            rows hold z_i in A only (input-consistency rows); M = 2^log_m
+//            rows; z_0 = 1, z_1..z_64 seeded (first num_public are public).
+//   setup    QAP at tau: L_j(tau) = (tau^M - 1) w^j / (M (tau - w^j)),
+//            A_i(tau) = sum_j A[j][i] L_j(tau) (likewise B, C) on the host;
+//            key points k*G by the GPU fixed-base comb and kept in HBM:
+//            A_i G1, B_i G1, B_i G2, (beta A_i + alpha B_i + C_i)/delta G1
+//            (i > num_public), tau^j Z(tau)/delta G1 (j < M-1).
+//   prove    SpMV (A z, B z, C z) -> 3 iNTT -> 3 coset NTT ->
+//            (A B - C)/(g^M - 1) -> coset iNTT = h -> MSMs over z and h ->
+//            phase-6 assembly on the host (groth16.cuh:210-215):
+//            A = alpha1 + sum z_i A_i + r delta1
+//            B = beta2 + sum z_i B2_i + s delta2,  B1 = beta1 + sum z_i B1_i + s delta1
+//            C = sum_{i>l} z_i L_i + sum_j h_j H_j + s A + r B1 - r s delta1
+// Streams: 2 circuit structure, 3 witness inputs, 4 toxic waste
+// (tau, alpha, beta, gamma, delta), 5 prover randomness (r, s).
+#include <omp.h>
+#include <vector>
+#include "common.hpp"
+#include "host_ec.hpp"
+#include "msm.hpp"
+#include "synth.hpp"
+
+using namespace hzk;
+using host::Fr;
+using host::Fq;
+using host::G1x;
+using host::G2x;
+
+namespace {
+
+const int NUM_INPUTS = 64;
+
+Fr fr_u64(uint64_t v) {
+    Fr t{{v, 0, 0, 0}};
+    Fr r2;
+    std::memcpy(r2.v, host::FR.r2, 32);
+    return t * r2;
+}
+Fr fr_draw(synth::Xoshiro& g) {
+    Fr r;
+    synth::draw(g, synth::FR_MOD, r.v);
+    return r;
+}
+Fr fr_root(int k) {
+    static const uint64_t t[4] = {0x9b9709143e1f593fULL, 0x181585d2833e8487ULL, 0x131a029b85045b68ULL, 0x000000030644e72eULL};
+    Fr w = fr_u64(5).pow(t);
+    for (int i = 28; i > k; --i) w = w.sqr();
+    return w;
+}
+Fr fr_pow_u64(Fr b, uint64_t e) {
+    Fr acc = Fr::one();
+    while (e) {
+        if (e & 1) acc = acc * b;
+        b = b.sqr();
+        e >>= 1;
+    }
+    return acc;
+}
+
+struct HostCsr {
+    std::vector<uint64_t> row_ptr, col;
+    std::vector<Fr> val;
+};
+
+}  // namespace
+
+struct hzk_g16 {
+    int log_m = 0;
+    size_t M = 0, num_vars = 0, num_public = 0, num_gates = 0;
+    HostCsr m[3];
+    // device copies
+    void* d_val[3] = {nullptr, nullptr, nullptr};
+    uint64_t* d_col[3] = {nullptr, nullptr, nullptr};
+    uint64_t* d_rp[3] = {nullptr, nullptr, nullptr};
+    // proving key in HBM (affine)
+    void *A1 = nullptr, *B1 = nullptr, *B2 = nullptr, *L1 = nullptr, *H1 = nullptr;
+    G1x alpha1, beta1, delta1;
+    G2x beta2, gamma2, delta2;
+    std::vector<G1x> IC;   // verifying-key input commitments
+    // per-proof device buffers
+    void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *res = nullptr, *zinv = nullptr;
+    std::vector<void*> owned;
+    int device = 0;
+};
+
+namespace {
+
+void build_circuit(hzk_g16* g, uint64_t seed) {
+    synth::Xoshiro rng(seed, 2);
+    HostCsr &A = g->m[0], &B = g->m[1], &C = g->m[2];
+    for (size_t gate = 0; gate < g->num_gates; ++gate) {
+        uint64_t v = 1 + NUM_INPUTS + gate;
+        for (HostCsr* mm : {&A, &B}) {
+            mm->row_ptr.push_back(mm->col.size());
+            int k = (int)(rng.next() % 3) + 1;
+            for (int t = 0; t < k; ++t) {
+                mm->col.push_back(rng.next() % v);
+                mm->val.push_back(fr_draw(rng));
+            }
+        }
+        C.row_ptr.push_back(C.col.size());
+        C.col.push_back(v);
+        C.val.push_back(Fr::one());
+    }
+    for (size_t i = 0; i <= g->num_public; ++i) {
+        A.row_ptr.push_back(A.col.size());
+        A.col.push_back(i);
+        A.val.push_back(Fr::one());
+        B.row_ptr.push_back(B.col.size());
+        C.row_ptr.push_back(C.col.size());
+    }
+}
+
+Fr row_dot(const HostCsr& m, size_t rows, size_t r, const Fr* z) {
+    size_t b = m.row_ptr[r], e = r + 1 < rows ? m.row_ptr[r + 1] : m.col.size();
+    Fr acc = Fr::zero();
+    for (size_t j = b; j < e; ++j) acc = acc + z[m.col[j]] * m.val[j];
+    return acc;
+}
+
+template <class T>
+int dev_upload(hzk_ctx* ctx, hzk_g16* g, const T* src, size_t bytes, void** out) {
+    void* d = nullptr;
+    HZK_HIP(ctx, hipMalloc(&d, bytes ? bytes : 1));
+    g->owned.push_back(d);
+    if (bytes) HZK_HIP(ctx, hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    *out = d;
+    return 0;
+}
+
+int dev_alloc(hzk_ctx* ctx, hzk_g16* g, size_t bytes, void** out) {
+    void* d = nullptr;
+    HZK_HIP(ctx, hipMalloc(&d, bytes ? bytes : 1));
+    g->owned.push_back(d);
+    *out = d;
+    return 0;
+}
+
+// k_i * G for host scalars k into a new device array of affine points
+template <bool G2>
+int key_points(hzk_ctx* ctx, hzk_g16* g, const std::vector<Fr>& k, void** out) {
+    size_t n = k.size();
+    int rc = dev_alloc(ctx, g, n * (G2 ? 128 : 64), out);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    void* ds = nullptr;
+    HZK_HIP(ctx, hipMalloc(&ds, n * 32));
+    HZK_HIP(ctx, hipMemcpy(ds, k.data(), n * 32, hipMemcpyHostToDevice));
+    rc = G2 ? hzk_fixbase_g2(ctx, ds, n, *out) : hzk_fixbase_g1(ctx, ds, n, *out);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ds);
+    return rc;
+}
+
+int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
+    build_circuit(g, seed);
+    // device CSR
+    for (int i = 0; i < 3; ++i) {
+        int rc = dev_upload(ctx, g, g->m[i].val.data(), g->m[i].val.size() * 32, &g->d_val[i]);
+        if (!rc) rc = dev_upload(ctx, g, g->m[i].col.data(), g->m[i].col.size() * 8, (void**)&g->d_col[i]);
+        if (!rc) rc = dev_upload(ctx, g, g->m[i].row_ptr.data(), g->m[i].row_ptr.size() * 8, (void**)&g->d_rp[i]);
+        if (rc) return rc;
+    }
+    // toxic waste
+    synth::Xoshiro tw(seed, 4);
+    Fr tau = fr_draw(tw), alpha = fr_draw(tw), beta = fr_draw(tw), gamma = fr_draw(tw), delta = fr_draw(tw);
+    const size_t M = g->M, N1 = g->num_vars, l = g->num_public;
+    // Lagrange basis at tau
+    Fr w = fr_root(g->log_m);
+    Fr Zt = fr_pow_u64(tau, M) - Fr::one();
+    Fr scale = Zt * fr_u64(M).inverse();
+    std::vector<Fr> L(M), den(M), pre(M);
+    Fr wj = Fr::one();
+    for (size_t j = 0; j < M; ++j) { L[j] = wj; den[j] = tau - wj; wj = wj * w; }
+    Fr acc = Fr::one();
+    for (size_t j = 0; j < M; ++j) { pre[j] = acc; acc = acc * den[j]; }
+    HZK_CHECK(ctx, !acc.is_zero(), HZK_ERR_INVALID, "setup: tau lies in the evaluation domain");
+    Fr inv = acc.inverse();
+    for (size_t j = M; j-- > 0;) { Fr dj = inv * pre[j]; inv = inv * den[j]; L[j] = L[j] * dj * scale; }
+    std::vector<Fr> At(N1, Fr::zero()), Bt(N1, Fr::zero()), Ct(N1, Fr::zero());
+    std::vector<Fr>* outs[3] = {&At, &Bt, &Ct};
+    for (int i = 0; i < 3; ++i) {
+        const HostCsr& mm = g->m[i];
+        for (size_t r = 0; r < M; ++r) {
+            size_t b = mm.row_ptr[r], e = r + 1 < M ? mm.row_ptr[r + 1] : mm.col.size();
+            for (size_t j = b; j < e; ++j) (*outs[i])[mm.col[j]] = (*outs[i])[mm.col[j]] + mm.val[j] * L[r];
+        }
+    }
+    Fr dinv = delta.inverse(), ginv = gamma.inverse();
+    std::vector<Fr> K(N1 - l - 1), IC(l + 1), H(M - 1);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)N1; ++i) {
+        Fr v = beta * At[i] + alpha * Bt[i] + Ct[i];
+        if ((size_t)i <= l) IC[i] = v * ginv;
+        else K[i - l - 1] = v * dinv;
+    }
+    Fr tp = Zt * dinv;
+    for (size_t j = 0; j + 1 < M; ++j) { H[j] = tp; tp = tp * tau; }
+    int rc = key_points<false>(ctx, g, At, &g->A1);
+    if (!rc) rc = key_points<false>(ctx, g, Bt, &g->B1);
+    if (!rc) rc = key_points<true>(ctx, g, Bt, &g->B2);
+    if (!rc) rc = key_points<false>(ctx, g, K, &g->L1);
+    if (!rc) rc = key_points<false>(ctx, g, H, &g->H1);
+    if (rc) return rc;
+    auto canon = [](const Fr& x, uint64_t c[4]) { x.canonical(c); };
+    uint64_t c[4];
+    G1x G = host::g1_generator();
+    G2x G2 = host::g2_generator();
+    canon(alpha, c); g->alpha1 = G.mul(c);
+    canon(beta, c); g->beta1 = G.mul(c); g->beta2 = G2.mul(c);
+    canon(delta, c); g->delta1 = G.mul(c); g->delta2 = G2.mul(c);
+    canon(gamma, c); g->gamma2 = G2.mul(c);
+    g->IC.resize(l + 1);
+    for (size_t i = 0; i <= l; ++i) { canon(IC[i], c); g->IC[i] = G.mul(c); }
+    // per-proof buffers and 1/(g^M - 1) for the coset division
+    rc = dev_alloc(ctx, g, N1 * 32, &g->z);
+    if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->a);
+    if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->b);
+    if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->c);
+    if (!rc) rc = dev_alloc(ctx, g, 5 * 192, &g->res);
+    if (rc) return rc;
+    Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
+    return dev_upload(ctx, g, zi.v, 32, &g->zinv);
+}
+
+}  // namespace
+
+namespace hzk {
+
+// Device phases of one proof over z already in g->z: SpMV, H, five MSMs.
+// Writes five normalised Jacobian points into host res[5 * 192].
+int g16_device_phases(hzk_ctx* ctx, hzk_g16* g, char* res) {
+    const size_t M = g->M, N1 = g->num_vars, l = g->num_public;
+    hzk_csr cs[3];
+    for (int i = 0; i < 3; ++i)
+        cs[i] = {g->d_val[i], g->d_col[i], g->d_rp[i], M, N1, g->m[i].col.size()};
+    int rc = spmv3(ctx, &cs[0], &cs[1], &cs[2], g->z, g->a, g->b, g->c);
+    for (void* v : {g->a, g->b, g->c}) {
+        if (!rc) rc = ntt_standard_raw(ctx, v, M, 1);
+        if (!rc) rc = coset_ntt_raw(ctx, v, M, 0);
+    }
+    if (!rc) rc = pointwise_abc(ctx, M, g->a, g->b, g->c, g->zinv);
+    if (!rc) rc = coset_ntt_raw(ctx, g->a, M, 1);
+    if (rc) return rc;
+    char* d = (char*)g->res;
+    rc = msm_g1(ctx, g->z, g->A1, N1, 0, d);
+    if (!rc) rc = msm_g1(ctx, g->z, g->B1, N1, 0, d + 96);
+    if (!rc) rc = msm_g2(ctx, g->z, g->B2, N1, 0, d + 192);
+    if (!rc) rc = msm_g1(ctx, (char*)g->z + (l + 1) * 32, g->L1, N1 - l - 1, 0, d + 384);
+    if (!rc) rc = msm_g1(ctx, g->a, g->H1, M - 1, 0, d + 480);
+    if (rc) return rc;
+    HZK_HIP(ctx, hipMemcpyAsync(res, g->res, 576, hipMemcpyDeviceToHost, ctx->stream));
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// Phase 6 on the host; proof = A (64 B) | B (128 B) | C (64 B), affine.
+void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof) {
+    Fr r = Fr::load(rs), s = Fr::load((const char*)rs + 32);
+    auto jac1 = [](const char* p) {   // normalised Jacobian (x, y, 1) or (0, 1, 0)
+        Fq z = Fq::load(p + 64);
+        if (z.is_zero()) return G1x::inf();
+        return G1x::from_affine(Fq::load(p), Fq::load(p + 32));
+    };
+    auto jac2 = [](const char* p) {
+        host::Fq2 z = host::Fq2::load(p + 128);
+        if (z.is_zero()) return G2x::inf();
+        return G2x::from_affine(host::Fq2::load(p), host::Fq2::load(p + 64));
+    };
+    G1x sA = jac1(res), sB1 = jac1(res + 96), sL = jac1(res + 384), sH = jac1(res + 480);
+    G2x sB2 = jac2(res + 192);
+    uint64_t rc[4], sc[4], rsc[4];
+    r.canonical(rc);
+    s.canonical(sc);
+    (r * s).canonical(rsc);
+    G1x A = g->alpha1.add(sA).add(g->delta1.mul(rc));
+    G2x B = g->beta2.add(sB2).add(g->delta2.mul(sc));
+    G1x B1 = g->beta1.add(sB1).add(g->delta1.mul(sc));
+    G1x C = sL.add(sH).add(A.mul(sc)).add(B1.mul(rc)).add(g->delta1.mul(rsc).neg());
+    char* out = (char*)proof;
+    A.store_affine(out);
+    B.store_affine(out + 64);
+    C.store_affine(out + 192);
+}
+
+}  // namespace hzk
+
+extern "C" {
+
+int hzk_g16_setup(hzk_ctx* ctx, uint64_t seed, int log_m, int num_public, hzk_g16** out) {
+    if (!ctx || !out) return HZK_ERR_INVALID;
+    *out = nullptr;
+    HZK_CHECK(ctx, log_m >= 4 && log_m <= 24, HZK_ERR_INVALID, "g16_setup: log_m %d out of [4, 24]", log_m);
+    HZK_CHECK(ctx, num_public >= 0 && num_public <= NUM_INPUTS, HZK_ERR_INVALID, "g16_setup: num_public %d", num_public);
+    hzk_g16* g = new hzk_g16();
+    g->log_m = log_m;
+    g->M = (size_t)1 << log_m;
+    g->num_public = num_public;
+    g->num_gates = g->M - (num_public + 1);
+    g->num_vars = 1 + NUM_INPUTS + g->num_gates;
+    g->device = ctx->device;
+    int rc = setup(ctx, g, seed);
+    if (rc) {
+        hzk_g16_free(g);
+        return rc;
+    }
+    *out = g;
+    return HZK_OK;
+}
+
+void hzk_g16_free(hzk_g16* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    (void)hipDeviceSynchronize();
+    for (void* p : g->owned) (void)hipFree(p);
+    delete g;
+}
+
+int hzk_g16_info(const hzk_g16* g, uint64_t* out8) {
+    if (!g || !out8) return HZK_ERR_INVALID;
+    uint64_t kb = g->num_vars * (64 + 64 + 128) + (g->num_vars - g->num_public - 1) * 64 + (g->M - 1) * 64;
+    uint64_t v[8] = {g->M, g->num_vars, g->num_public, g->num_gates, g->m[0].col.size(), g->m[1].col.size(),
+                     g->m[2].col.size(), kb};
+    std::memcpy(out8, v, sizeof v);
+    return HZK_OK;
+}
+
+int hzk_g16_witness(const hzk_g16* g, uint64_t wseed, void* z_host) {
+    if (!g || !z_host) return HZK_ERR_INVALID;
+    Fr* z = (Fr*)z_host;
+    z[0] = Fr::one();
+    synth::Xoshiro rng(wseed, 3);
+    for (int i = 1; i <= NUM_INPUTS; ++i) z[i] = fr_draw(rng);
+    for (size_t gate = 0; gate < g->num_gates; ++gate)
+        z[1 + NUM_INPUTS + gate] = row_dot(g->m[0], g->M, gate, z) * row_dot(g->m[1], g->M, gate, z);
+    return HZK_OK;
+}
+
+int hzk_g16_rs(uint64_t wseed, void* rs_host) {
+    if (!rs_host) return HZK_ERR_INVALID;
+    synth::Xoshiro rng(wseed, 5);
+    Fr r = fr_draw(rng), s = fr_draw(rng);
+    std::memcpy(rs_host, r.v, 32);
+    std::memcpy((char*)rs_host + 32, s.v, 32);
+    return HZK_OK;
+}
+
+int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_host, void* proof_host) {
+    if (!ctx || !g) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, z_host && rs_host && proof_host, HZK_ERR_INVALID, "g16_prove: null argument");
+    HZK_HIP(ctx, hipMemcpyAsync(g->z, z_host, g->num_vars * 32, hipMemcpyHostToDevice, ctx->stream));
+    char res[576];
+    int rc = g16_device_phases(ctx, g, res);
+    if (rc) return rc;
+    g16_assemble(g, res, rs_host, proof_host);
+    return HZK_OK;
+}
+
+// Device pointer and element count of a proving-key array (tests, serving):
+// which 0 A1, 1 B1, 2 B2 (G2), 3 L1, 4 H1.
+int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count) {
+    if (!g || !dev_ptr || !count) return HZK_ERR_INVALID;
+    switch (which) {
+        case 0: *dev_ptr = g->A1; *count = g->num_vars; break;
+        case 1: *dev_ptr = g->B1; *count = g->num_vars; break;
+        case 2: *dev_ptr = g->B2; *count = g->num_vars; break;
+        case 3: *dev_ptr = g->L1; *count = g->num_vars - g->num_public - 1; break;
+        case 4: *dev_ptr = g->H1; *count = g->M - 1; break;
+        default: return HZK_ERR_INVALID;
+    }
+    return HZK_OK;
+}
+
+}  // extern "C"
