@@ -63,6 +63,33 @@ def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
                       f"(test/msm-check.cu:46-55 semantics), {el:.1f} s"}
 
 
+def groth16_plain(ctx, log_m, proofs):
+    """Plain one-proof-at-a-time GPU prover on the synthetic circuit (seed 2)."""
+    import liblightzk_amd as z
+    t = time.perf_counter()
+    g = z.Groth16(ctx, seed=2, log_m=log_m, num_public=8)
+    setup_s = time.perf_counter() - t
+    wits = [(g.witness(100 + i), g.rs(100 + i)) for i in range(proofs + 1)]
+    g.prove(*wits[0])   # warm-up (tables, scratch)
+    lat = []
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for zz, rs in wits[1:]:
+        a = time.perf_counter()
+        g.prove(zz, rs)
+        lat.append(time.perf_counter() - a)
+    el = time.perf_counter() - t0
+    rep = ctx.timing_report()
+    ctx.timing(False)
+    g.close()
+    lat.sort()
+    return {"config": f"m=2^{log_m} multiplication-gate circuit, {g.info['num_vars']} variables",
+            "proofs": proofs, "proofs_per_s": proofs / el, "p50_ms": lat[len(lat) // 2] * 1e3,
+            "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))] * 1e3, "setup_s": setup_s,
+            "kernels_ms_per_proof": {k: v[0] / proofs for k, v in rep.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +98,8 @@ def main():
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--g16-log-m", type=int, default=18)
+    ap.add_argument("--g16-proofs", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU only "
@@ -144,6 +173,8 @@ def main():
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in rep.items()},
         "roofline": roofline,
     }
+    if args.g16_proofs > 0:
+        out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scalars, d_b, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -260,12 +260,13 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.NB = 1u << (c - 1);
-    // segment length: long enough to amortise the segment bookkeeping, short
-    // enough that about 2^17 lanes share the work (per-lane point additions
-    // are latency-bound below that)
+    // segment length: about 2^18 lanes share the n*W additions (fewer leave
+    // SIMDs idle: one lane's chain of point additions is latency-bound), and
+    // at 2^24 points a typical bucket (~512 entries) stays one segment, so
+    // only oversized buckets pay for the segment merge
     size_t L = n * (size_t)p.W;
-    uint32_t K = 64;
-    while (K > 8 && L / K < ((size_t)1 << 17)) K >>= 1;
+    uint32_t K = 8;
+    while (K < 4096 && L / (2 * K) >= ((size_t)1 << 18)) K <<= 1;
     p.K = K;
     uint32_t g = p.NB / 512;
     if (g < 4) g = 4;

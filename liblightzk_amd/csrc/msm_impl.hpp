@@ -50,7 +50,21 @@ __global__ void __launch_bounds__(256) k_msm_accumulate(const affine_t<F>* __res
     seg2key[s] = k;
 }
 
-// one level of the in-place K2-ary tree over each key's segment sums
+// keys with 2..K2 segments: one lane per key sums them into the first slot
+template <class F>
+__global__ void __launch_bounds__(256) k_msm_merge_small(const uint32_t* __restrict__ segbase, uint32_t nk,
+                                                         xyzz_t<F>* __restrict__ part) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    uint32_t b0 = segbase[k], cnt = segbase[k + 1] - b0;
+    if (cnt < 2 || cnt > K2) return;
+    xyzz_t<F> acc = part[b0];
+    for (uint32_t i = 1; i < cnt; ++i) acc.add(part[b0 + i]);
+    part[b0] = acc;
+}
+
+// keys with more than K2 segments (oversized buckets): one level of an
+// in-place K2-ary tree over the key's segment sums
 template <class F>
 __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ segbase, const uint32_t* __restrict__ seg2key,
                                                    uint32_t nseg, uint32_t stride, xyzz_t<F>* __restrict__ part) {
@@ -58,7 +72,7 @@ __global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ 
     if (s >= nseg) return;
     uint32_t k = seg2key[s];
     uint32_t b0 = segbase[k], cnt = segbase[k + 1] - b0, rel = s - b0;
-    if (cnt <= stride || rel % (stride * K2) != 0) return;
+    if (cnt <= K2 || cnt <= stride || rel % (stride * K2) != 0) return;
     xyzz_t<F> acc = part[s];
     for (int i = 1; i < K2; ++i) {
         uint32_t r = rel + i * stride;
@@ -193,7 +207,11 @@ int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size
                                sorted, offs, segbase, nk, S, p.K, part, seg2key);
         }
         uint32_t maxseg = (maxc + p.K - 1) / p.K;
-        for (uint32_t stride = 1; stride < maxseg; stride *= K2) {
+        if (maxseg > 1) {
+            TimedLaunch tl(ctx, "msm_merge");
+            hipLaunchKernelGGL(k_msm_merge_small<F>, dim3(blocks_for(nk, 256)), dim3(256), 0, st, segbase, nk, part);
+        }
+        for (uint32_t stride = 1; maxseg > (uint32_t)K2 && stride < maxseg; stride *= K2) {
             TimedLaunch tl(ctx, "msm_merge");
             hipLaunchKernelGGL(k_msm_merge<F>, dim3(blocks_for(S, 256)), dim3(256), 0, st, segbase, seg2key, S, stride, part);
         }
