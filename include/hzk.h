@@ -72,6 +72,23 @@ int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
                size_t win_siz, void* dest);
 
+/* Batched MSMs of uneven sizes (no reference counterpart: the reference runs
+ * one MSM per call). Every stage runs as one launch over the whole batch; with
+ * HZK_BATCH_GRAPH the device stages are captured into a hipGraph on first use
+ * of a batch shape (sizes and plans) and replayed afterwards. Each result is
+ * bit-identical to the same instance run alone through hzk_msm_g1/_g2. */
+typedef struct hzk_msm_desc {
+    const void* scalars;   /* device Fr[n] */
+    const void* bases;     /* device affine points */
+    size_t n;
+    void* dest;            /* Jacobian result (device, or host with HZK_BATCH_HOST_DEST) */
+    uint32_t win_siz;      /* 0 = autotuned */
+} hzk_msm_desc;
+#define HZK_BATCH_HOST_DEST 1u
+#define HZK_BATCH_GRAPH 2u
+int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
+int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
+
 /* statistics of the last MSM on this context: {bucket entries (nonzero
  * digits), accumulation segments, largest bucket, window bits c, windows} */
 int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5);

@@ -30,6 +30,12 @@ G2_JAC_BYTES = 192
 FIELD_BYTES = 32
 
 
+class MsmDesc(ctypes.Structure):
+    """hzk_msm_desc: one instance of a batched MSM."""
+    _fields_ = [("scalars", ctypes.c_void_p), ("bases", ctypes.c_void_p), ("n", ctypes.c_size_t),
+                ("dest", ctypes.c_void_p), ("win_siz", ctypes.c_uint32)]
+
+
 class DeviceBuffer:
     """Owned device allocation (xpu::vector with mem_policy::device_only)."""
 
@@ -177,6 +183,15 @@ class Context:
 
     def coset_ntt(self, coeffs, deg, inverse=False):
         self._check(load().hzk_coset_ntt(self.handle, int(coeffs), deg, 1 if inverse else 0))
+
+    def msm_batch(self, group, jobs, host_dest=False, graph=False):
+        """jobs: list of (scalars, bases, n, dest[, win_siz]); device pointers (dest host if host_dest)."""
+        arr = (MsmDesc * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = MsmDesc(int(j[0]), int(j[1]), j[2], int(j[3]), j[4] if len(j) > 4 else 0)
+        flags = (1 if host_dest else 0) | (2 if graph else 0)
+        fn = load().hzk_msm_g1_batch if group == 1 else load().hzk_msm_g2_batch
+        self._check(fn(self.handle, arr, len(jobs), flags))
 
     def msm_last_stats(self):
         a = (ctypes.c_uint32 * 5)()

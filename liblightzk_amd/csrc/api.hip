@@ -3,6 +3,7 @@
 // include/hzk.h).
 #include <cstring>
 #include <mutex>
+#include <vector>
 #include "common.hpp"
 #include "curve.hpp"
 #include "msm.hpp"
@@ -28,6 +29,7 @@ int ensure_scratch(hzk_ctx* ctx, size_t bytes) {
         --ctx->fail_alloc_after;
     }
     if (ctx->scratch_bytes >= bytes) return 0;
+    ++ctx->scratch_generation;
     if (ctx->scratch) {
         HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         HZK_HIP(ctx, hipFree(ctx->scratch));
@@ -142,7 +144,10 @@ int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->lds_per_block = prop.sharedMemPerBlock;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->lds_per_block = prop.sharedMemPerBlock;
+        ctx->num_cus = prop.multiProcessorCount;
+    }
     if (stream) {
         ctx->stream = (hipStream_t)stream;
     } else {
@@ -158,6 +163,8 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drain_timing(ctx);
+    for (auto& kv : ctx->graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     ntt_release(ctx);
@@ -250,6 +257,22 @@ int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
     if (!ctx) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
     return msm_g2(ctx, scalars, bases, n, win_siz, dest);
+}
+
+static int msm_batch_entry(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags, bool g2) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, d && count > 0, HZK_ERR_INVALID, "msm batch: empty");
+    HZK_CHECK(ctx, (flags & ~(HZK_BATCH_HOST_DEST | HZK_BATCH_GRAPH)) == 0, HZK_ERR_INVALID, "msm batch: bad flags %u", flags);
+    std::vector<MsmJob> jobs(count);
+    for (size_t i = 0; i < count; ++i) jobs[i] = {d[i].scalars, d[i].bases, d[i].n, d[i].dest, d[i].win_siz};
+    int host = (flags & HZK_BATCH_HOST_DEST) ? 1 : 0, graph = (flags & HZK_BATCH_GRAPH) ? 1 : 0;
+    return g2 ? msm_batch_g2(ctx, jobs.data(), count, host, graph) : msm_batch_g1(ctx, jobs.data(), count, host, graph);
+}
+int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags) {
+    return msm_batch_entry(ctx, d, count, flags, false);
+}
+int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags) {
+    return msm_batch_entry(ctx, d, count, flags, true);
 }
 
 int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5) {

@@ -8,8 +8,21 @@
 #include <vector>
 #include "../../include/hzk.h"
 
+struct hzk_graph_entry {
+    hipGraphExec_t exec = nullptr;
+    uint64_t generation = 0;
+};
+
+// autotuned MSM parameters for a size class (log2 n); c = 0: use the heuristic
+struct hzk_tune_entry {
+    int c = 0;
+    uint32_t K = 0;
+    uint32_t g = 0;
+};
+
 struct hzk_ctx {
     int device = 0;
+    int num_cus = 256;
     size_t lds_per_block = 65536;   // device limit, read at creation
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -17,6 +30,9 @@ struct hzk_ctx {
     // scratch arena: grown on demand, reused across calls, freed at destroy
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    uint64_t scratch_generation = 0;   // bumped on every reallocation (invalidates captured graphs)
+    // captured device-stage graphs keyed by batch shape
+    std::map<std::string, hzk_graph_entry> graphs;
     // pinned host staging for small results
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -35,6 +51,7 @@ struct hzk_ctx {
     uint32_t last_msm[5] = {0, 0, 0, 0, 0};
     // NTT twiddle / coset tables for the last transform size (ntt.hip)
     void* ntt_cache = nullptr;
+    hzk_tune_entry tune[32];
     // autotune overrides (0 = automatic)
     int force_window = 0;
     int force_seg = 0;

@@ -1,4 +1,5 @@
-// Pippenger multi-scalar multiplication for alt_bn128 G1 and G2 on gfx950.
+// Pippenger multi-scalar multiplication for alt_bn128 G1 and G2 on gfx950:
+// scalar-side stages, batched over any number of MSM instances.
 //
 // Replaces the reference's operator (include/operator/msm.cuh:77-248: one
 // counting sort of n*win_cnt (bucket, point) pairs into 2^c buckets with
@@ -14,23 +15,21 @@
 //              (2^(c-1) counters, <= 128 KiB), one coalesced row per tile. No
 //              device-scope atomics: on a multi-XCD part they resolve past the
 //              per-XCD L2 and ran at ~5 G/s, slower than the whole accumulation.
-//  3. scan     per key a prefix over tiles, then a device-wide exclusive scan
-//              of (count, ceil(count/K)) -> bucket offsets and level-1 segment
-//              bases; totals and the largest bucket go back to the host in one
-//              small copy.
+//  3. scan     per key a prefix over tiles, then an exclusive scan of
+//              (count, ceil(count/K)) -> bucket offsets and level-1 segment
+//              bases; per-instance totals land in device stats, so no stage
+//              waits for the host.
 //  4. scatter  same (tile, window) workgroups: LDS cursors seeded from the
 //              scan, LDS atomics hand out slots, point index | sign<<31 is
 //              written into its bucket.
-//  5. accumulate  lane per segment of <= K entries of one bucket: mixed
-//              additions of affine bases into an XYZZ accumulator. Buckets of any
-//              size are split into segments, so a hot bucket never serialises.
-//  6. merge    strided in-place tree over each bucket's segment sums (K2-ary),
-//              only as many levels as the largest bucket needs.
-//  7. bucket reduction  lane per run of g buckets: running sums give
-//              sum (d - lo) B_d and sum B_d; the segment's weight lo is applied
-//              by double-and-add; a block per window then tree-sums segments.
-//  8. window combine on the host (Horner with c doublings per window) and
+//  5-8 (msm_impl.hpp) accumulate, merge, bucket reduction, window sums;
+//  9. window combine on the host (Horner with c doublings per window) and
 //              normalisation; see host_ec.hpp for why this tail runs on the CPU.
+//
+// Every stage is one launch over the whole batch: each instance owns a
+// contiguous range of every work space (MsmSpace), and a lane finds its
+// instance by binary search over the range starts. A single MSM is a batch of
+// one; a batch of uneven sizes fills the GPU where one small MSM cannot.
 #include <algorithm>
 #include <cstring>
 #include "common.hpp"
@@ -40,109 +39,112 @@
 namespace hzk {
 
 static constexpr uint32_t DIGIT_NEG = 0x80000000u;
+static constexpr int SCAN_ITEMS = 4;
+static constexpr int SCAN_THREADS = 256;
+static_assert(SCAN_ITEMS * SCAN_THREADS == MSM_SCAN_TILE, "scan tile");
 
-// ------------------------------------------------------------------ digits
-template <int C>
-HZK_DEV uint32_t window_bits(const fr_t& s, int j) {
-    constexpr uint32_t MASK = (1u << C) - 1;
-    const int lo = j * C;
-    const int w = lo >> 5, sh = lo & 31;
-    uint64_t v = s.l[w];
-    if (w + 1 < 8) v |= (uint64_t)s.l[w + 1] << 32;
-    return (uint32_t)(v >> sh) & MASK;
+// largest i < ninst with beg[i] <= gid (beg[0] = 0 <= gid < beg[ninst])
+HZK_DEV uint32_t find_inst(const uint64_t* __restrict__ beg, uint32_t ninst, uint64_t gid) {
+    uint32_t lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (beg[mid] <= gid) lo = mid; else hi = mid;
+    }
+    return lo;
 }
+HZK_DEV const uint64_t* space(const MsmBatchDev& b, int s) { return b.begin + (size_t)s * (b.ninst + 1); }
 
+// ------------------------------------------------------------------ 1. digits
 // digits[j * n + i] = |d_ij| | sign << 31, or 0 for a zero digit
-template <int C>
-__global__ void __launch_bounds__(256) k_msm_digits(const fr_t* __restrict__ scalars, uint32_t n,
-                                                   uint32_t* __restrict__ digits) {
-    constexpr int W = (255 + C - 1) / C;
-    constexpr uint32_t NB = 1u << (C - 1);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    fr_t s = scalars[i].from_mont();
-    uint32_t carry = 0;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-        uint32_t raw = window_bits<C>(s, j) + carry;
-        uint32_t d;
-        if (raw > NB) { carry = 1; d = (2 * NB - raw) | DIGIT_NEG; }
-        else { carry = 0; d = raw; }
-        if ((d & ~DIGIT_NEG) == 0) d = 0;
-        digits[(size_t)j * n + i] = d;
-    }
-}
-
-// tile t of window j covers scalars [t*tile, min(n, (t+1)*tile))
-__global__ void __launch_bounds__(1024) k_msm_hist(const uint32_t* __restrict__ digits, uint32_t n, uint32_t tile,
-                                                   uint32_t NB, uint32_t nk, uint32_t* __restrict__ hist) {
-    extern __shared__ uint32_t cnt[];
-    const uint32_t t = blockIdx.x, j = blockIdx.y;
-    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) cnt[b] = 0;
-    __syncthreads();
-    const uint32_t i0 = t * tile, i1 = min(n, i0 + tile);
-    const uint32_t* dj = digits + (size_t)j * n;
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        uint32_t d = dj[i] & ~DIGIT_NEG;
-        if (d) atomicAdd(&cnt[d - 1], 1u);
-    }
-    __syncthreads();
-    uint32_t* row = hist + (size_t)t * nk + (size_t)j * NB;
-    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) row[b] = cnt[b];
-}
-
-// per key: count = sum over tiles, hist[t][key] <- exclusive prefix over tiles
-__global__ void __launch_bounds__(256) k_msm_colscan(uint32_t* __restrict__ hist, uint32_t nk, uint32_t T,
-                                                     uint32_t* __restrict__ count) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nk) return;
-    uint32_t run = 0;
-    for (uint32_t t = 0; t < T; ++t) {
-        uint32_t v = hist[(size_t)t * nk + k];
-        hist[(size_t)t * nk + k] = run;
-        run += v;
-    }
-    count[k] = run;
-}
-
-__global__ void __launch_bounds__(1024) k_msm_scatter(const uint32_t* __restrict__ digits, uint32_t n, uint32_t tile,
-                                                      uint32_t NB, uint32_t nk, const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ offs, uint32_t* __restrict__ sorted) {
-    extern __shared__ uint32_t cur[];
-    const uint32_t t = blockIdx.x, j = blockIdx.y;
-    const uint32_t* row = hist + (size_t)t * nk + (size_t)j * NB;
-    for (uint32_t b = threadIdx.x; b < NB; b += blockDim.x) cur[b] = offs[j * NB + b] + row[b];
-    __syncthreads();
-    const uint32_t i0 = t * tile, i1 = min(n, i0 + tile);
-    const uint32_t* dj = digits + (size_t)j * n;
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        uint32_t d = dj[i];
-        uint32_t mag = d & ~DIGIT_NEG;
-        if (mag) {
-            uint32_t pos = atomicAdd(&cur[mag - 1], 1u);
-            sorted[pos] = i | (d & DIGIT_NEG);
+__global__ void __launch_bounds__(256) k_msm_digits(MsmBatchDev b, uint32_t* __restrict__ digits) {
+    __shared__ uint32_t limbs[256][9];
+    const uint64_t* beg = space(b, SP_DIGIT);
+    const uint64_t* ebeg = space(b, SP_ENTRY);
+    const uint64_t total = beg[b.ninst];
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t ii = find_inst(beg, b.ninst, gid);
+        const MsmInst& d = b.inst[ii];
+        uint32_t idx = (uint32_t)(gid - beg[ii]);
+        fr_t s = ((const fr_t*)d.scalars)[idx].from_mont();
+        uint32_t* my = limbs[threadIdx.x];
+        for (int w = 0; w < 8; ++w) my[w] = s.l[w];
+        my[8] = 0;
+        const uint32_t c = d.c, NB = d.NB, mask = (1u << c) - 1;
+        uint32_t carry = 0;
+        uint32_t* out = digits + ebeg[ii] + idx;
+        for (uint32_t j = 0; j < d.W; ++j) {
+            uint32_t lo = j * c, w = lo >> 5, sh = lo & 31;
+            uint64_t v = (uint64_t)my[w] | ((uint64_t)my[w + 1] << 32);
+            uint32_t raw = ((uint32_t)(v >> sh) & mask) + carry;
+            uint32_t dg;
+            if (raw > NB) { carry = 1; dg = (2 * NB - raw) | DIGIT_NEG; }
+            else { carry = 0; dg = raw; }
+            if ((dg & ~DIGIT_NEG) == 0) dg = 0;
+            out[(size_t)j * d.n] = dg;
         }
     }
 }
 
-// ------------------------------------------------------------------ scan
-// packed value: low 32 bits = bucket count, high 32 bits = ceil(count / K)
-static constexpr int SCAN_ITEMS = 4;
-static constexpr int SCAN_THREADS = 256;
-static constexpr int SCAN_TILE = SCAN_ITEMS * SCAN_THREADS;
+// ------------------------------------------------------------------ 2. histogram
+// workgroup (t, j) of an instance counts the digits of scalars [t*tile, (t+1)*tile) in window j
+__global__ void __launch_bounds__(1024) k_msm_hist(MsmBatchDev b, const uint32_t* __restrict__ digits, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t cnt[];
+    const uint64_t* beg = space(b, SP_TILEWG);
+    uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
+    uint32_t t = local % d.T, j = local / d.T;
+    for (uint32_t x = threadIdx.x; x < d.NB; x += blockDim.x) cnt[x] = 0;
+    __syncthreads();
+    const uint32_t i0 = t * d.tile, i1 = min(d.n, i0 + d.tile);
+    const uint32_t* dj = digits + space(b, SP_ENTRY)[ii] + (size_t)j * d.n;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        uint32_t m = dj[i] & ~DIGIT_NEG;
+        if (m) atomicAdd(&cnt[m - 1], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
+    for (uint32_t x = threadIdx.x; x < d.NB; x += blockDim.x) row[x] = cnt[x];
+}
 
+// ------------------------------------------------------------------ 3. scans
+// per key: count = sum over tiles; hist[t][key] <- exclusive prefix over tiles
+__global__ void __launch_bounds__(256) k_msm_colscan(MsmBatchDev b, uint32_t* __restrict__ hist, uint32_t* __restrict__ count) {
+    const uint64_t* beg = space(b, SP_KEY);
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= beg[b.ninst]) return;
+    uint32_t ii = find_inst(beg, b.ninst, gid);
+    const MsmInst& d = b.inst[ii];
+    uint32_t k = (uint32_t)(gid - beg[ii]);
+    if (k >= d.nk) { count[gid] = 0; return; }
+    uint32_t* h = hist + space(b, SP_HIST)[ii] + k;
+    uint32_t run = 0;
+    for (uint32_t t = 0; t < d.T; ++t) {
+        uint32_t v = h[(size_t)t * d.nk];
+        h[(size_t)t * d.nk] = run;
+        run += v;
+    }
+    count[gid] = run;
+}
+
+// packed value: low 32 bits = bucket count, high 32 bits = ceil(count / K)
 HZK_DEV uint64_t pack_count(uint32_t c, uint32_t K) { return ((uint64_t)((c + K - 1) / K) << 32) | c; }
 
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __restrict__ count, uint32_t nk, uint32_t K,
-                                                             uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ stats) {
+// one block per scan tile: tile sum and tile maximum
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(MsmBatchDev b, const uint32_t* __restrict__ count,
+                                                             uint64_t* __restrict__ tile_sum, uint64_t* __restrict__ tile_max) {
     __shared__ uint64_t red[SCAN_THREADS];
     __shared__ uint32_t redmax[SCAN_THREADS];
-    uint32_t base = blockIdx.x * SCAN_TILE;
+    const uint64_t* tbeg = space(b, SP_SCANTILE);
+    uint32_t ii = find_inst(tbeg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    uint64_t kbase = space(b, SP_KEY)[ii];
+    uint32_t base = (uint32_t)(blockIdx.x - tbeg[ii]) * MSM_SCAN_TILE;
     uint64_t s = 0;
     uint32_t mx = 0;
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + threadIdx.x + t * SCAN_THREADS;
-        if (k < nk) { uint32_t c = count[k]; s += pack_count(c, K); mx = max(mx, c); }
+        if (k <= d.nk) { uint32_t c = count[kbase + k]; s += pack_count(c, d.K); mx = max(mx, c); }
     }
     red[threadIdx.x] = s;
     redmax[threadIdx.x] = mx;
@@ -156,19 +158,23 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(const uint32_t* __r
     }
     if (threadIdx.x == 0) {
         tile_sum[blockIdx.x] = red[0];
-        tile_sum[gridDim.x + blockIdx.x] = redmax[0];
+        tile_max[blockIdx.x] = redmax[0];
     }
 }
 
-// single block: exclusive scan of the tile sums in place; totals and max to stats
-__global__ void __launch_bounds__(1024) k_scan_top(uint64_t* __restrict__ tile_sum, uint32_t ntiles, uint32_t* __restrict__ stats) {
+// one block per instance: exclusive scan of its tile sums; totals to stats
+__global__ void __launch_bounds__(1024) k_scan_top(MsmBatchDev b, uint64_t* __restrict__ tile_sum, const uint64_t* __restrict__ tile_max) {
     __shared__ uint64_t part[1024];
     __shared__ uint32_t pmax[1024];
+    const uint64_t* tbeg = space(b, SP_SCANTILE);
+    const uint32_t ii = blockIdx.x;
+    const uint64_t t0 = tbeg[ii];
+    const uint32_t ntiles = (uint32_t)(tbeg[ii + 1] - t0);
     uint32_t per = (ntiles + 1023) / 1024;
-    uint32_t b = threadIdx.x * per, e = min(ntiles, b + per);
+    uint32_t lo = threadIdx.x * per, hi = min(ntiles, lo + per);
     uint64_t s = 0;
     uint32_t mx = 0;
-    for (uint32_t i = b; i < e; ++i) { s += tile_sum[i]; mx = max(mx, (uint32_t)tile_sum[ntiles + i]); }
+    for (uint32_t i = lo; i < hi; ++i) { s += tile_sum[t0 + i]; mx = max(mx, (uint32_t)tile_max[t0 + i]); }
     part[threadIdx.x] = s;
     pmax[threadIdx.x] = mx;
     __syncthreads();
@@ -181,25 +187,30 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint64_t* __restrict__ tile_s
         __syncthreads();
     }
     uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-    for (uint32_t i = b; i < e; ++i) { uint64_t v = tile_sum[i]; tile_sum[i] = run; run += v; }
+    for (uint32_t i = lo; i < hi; ++i) { uint64_t v = tile_sum[t0 + i]; tile_sum[t0 + i] = run; run += v; }
     if (threadIdx.x == 1023) {
-        stats[0] = (uint32_t)part[1023];
-        stats[1] = (uint32_t)(part[1023] >> 32);
-        stats[2] = pmax[1023];
+        b.stats[4 * ii + 0] = (uint32_t)part[1023];
+        b.stats[4 * ii + 1] = (uint32_t)(part[1023] >> 32);
+        b.stats[4 * ii + 2] = pmax[1023];
+        b.stats[4 * ii + 3] = 0;
     }
 }
 
-// per tile: exclusive scan, write bucket offsets and segment bases
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __restrict__ count, uint32_t nk, uint32_t K,
+// per tile: exclusive scan; bucket offsets and segment bases for keys 0..nk
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(MsmBatchDev b, const uint32_t* __restrict__ count,
                                                              const uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ offs,
                                                              uint32_t* __restrict__ segbase) {
     __shared__ uint64_t part[SCAN_THREADS];
-    uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint64_t* tbeg = space(b, SP_SCANTILE);
+    uint32_t ii = find_inst(tbeg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    uint64_t kbase = space(b, SP_KEY)[ii];
+    uint32_t base = (uint32_t)(blockIdx.x - tbeg[ii]) * MSM_SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint64_t v[SCAN_ITEMS];
     uint64_t s = 0;
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + t;
-        v[t] = k < nk ? pack_count(count[k], K) : 0;
+        v[t] = k <= d.nk ? pack_count(count[kbase + k], d.K) : 0;
         s += v[t];
     }
     part[threadIdx.x] = s;
@@ -213,39 +224,85 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
     uint64_t run = tile_sum[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + t;
-        if (k <= nk) {
-            offs[k] = (uint32_t)run;
-            segbase[k] = (uint32_t)(run >> 32);
+        if (k <= d.nk) {
+            offs[kbase + k] = (uint32_t)run;
+            segbase[kbase + k] = (uint32_t)(run >> 32);
         }
         run += v[t];
     }
 }
 
-// ------------------------------------------------------------------ driver pieces
-template <int C>
-static void launch_digits(hzk_ctx* ctx, const fr_t* scalars, uint32_t n, uint32_t* digits) {
-    TimedLaunch tl(ctx, "msm_digits");
-    hipLaunchKernelGGL(k_msm_digits<C>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, scalars, n, digits);
-}
-typedef void (*digits_fn)(hzk_ctx*, const fr_t*, uint32_t, uint32_t*);
-template <int C>
-struct CTab {
-    static void fill(digits_fn* d) {
-        d[C] = launch_digits<C>;
-        CTab<C - 1>::fill(d);
+// ------------------------------------------------------------------ 4. scatter
+__global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint32_t* __restrict__ digits,
+                                                      const uint32_t* __restrict__ hist, const uint32_t* __restrict__ offs,
+                                                      uint32_t* __restrict__ sorted) {
+    extern __shared__ uint32_t cur[];
+    const uint64_t* beg = space(b, SP_TILEWG);
+    uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
+    uint32_t t = local % d.T, j = local / d.T;
+    const uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
+    const uint32_t* of = offs + space(b, SP_KEY)[ii] + (size_t)j * d.NB;
+    for (uint32_t x = threadIdx.x; x < d.NB; x += blockDim.x) cur[x] = of[x] + row[x];
+    __syncthreads();
+    const uint32_t i0 = t * d.tile, i1 = min(d.n, i0 + d.tile);
+    const uint64_t ebase = space(b, SP_ENTRY)[ii];
+    const uint32_t* dj = digits + ebase + (size_t)j * d.n;
+    uint32_t* out = sorted + ebase;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        uint32_t dg = dj[i];
+        uint32_t m = dg & ~DIGIT_NEG;
+        if (m) {
+            uint32_t pos = atomicAdd(&cur[m - 1], 1u);
+            out[pos] = i | (dg & DIGIT_NEG);
+        }
     }
-};
-template <>
-struct CTab<MSM_MIN_C - 1> {
-    static void fill(digits_fn*) {}
-};
-struct StageTables {
-    digits_fn digits[MSM_MAX_C + 1];
-    StageTables() { CTab<MSM_MAX_C>::fill(digits); }
-};
-static const StageTables& stage_tables() {
-    static StageTables t;
-    return t;
+}
+
+// ------------------------------------------------------------------ host side
+void msm_prepare() {
+    static bool done = false;
+    if (done) return;
+    done = true;
+    int bytes = (1 << (MSM_MAX_C - 1)) * 4;
+    (void)hipFuncSetAttribute((const void*)k_msm_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    (void)hipFuncSetAttribute((const void*)k_msm_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap) {
+    uint64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    return (unsigned)(g < cap ? g : cap);
+}
+
+void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t* tot, uint32_t max_nb,
+                              const MsmScalarBufs& bufs) {
+    hipStream_t st = ctx->stream;
+    const uint64_t ntiles = tot[SP_SCANTILE];
+    {
+        TimedLaunch tl(ctx, "msm_digits");
+        hipLaunchKernelGGL(k_msm_digits, dim3(grid_for(tot[SP_DIGIT], 256, 1u << 16)), dim3(256), 0, st, b, bufs.digits);
+    }
+    {
+        TimedLaunch tl(ctx, "msm_hist");
+        hipLaunchKernelGGL(k_msm_hist, dim3((unsigned)tot[SP_TILEWG]), dim3(1024), max_nb * 4, st, b, (const uint32_t*)bufs.digits,
+                           bufs.hist);
+    }
+    {
+        TimedLaunch tl(ctx, "msm_scan");
+        hipLaunchKernelGGL(k_msm_colscan, dim3(grid_for(tot[SP_KEY], 256, 1u << 30)), dim3(256), 0, st, b, bufs.hist, bufs.count);
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, st, b, (const uint32_t*)bufs.count,
+                           bufs.tiles, bufs.tiles + ntiles);
+        hipLaunchKernelGGL(k_scan_top, dim3(b.ninst), dim3(1024), 0, st, b, bufs.tiles, (const uint64_t*)(bufs.tiles + ntiles));
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, st, b, (const uint32_t*)bufs.count,
+                           (const uint64_t*)bufs.tiles, bufs.offs, bufs.segbase);
+    }
+    {
+        TimedLaunch tl(ctx, "msm_scatter");
+        hipLaunchKernelGGL(k_msm_scatter, dim3((unsigned)tot[SP_TILEWG]), dim3(1024), max_nb * 4, st, b,
+                           (const uint32_t*)bufs.digits, (const uint32_t*)bufs.hist, (const uint32_t*)bufs.offs, bufs.sorted);
+    }
 }
 
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
@@ -282,46 +339,19 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     return p;
 }
 
-// Allow the histogram / scatter workgroups the LDS the largest window needs.
-static void set_lds_limits() {
-    static bool done = false;
-    if (done) return;
-    done = true;
-    int bytes = (1 << (MSM_MAX_C - 1)) * 4;
-    (void)hipFuncSetAttribute((const void*)k_msm_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_msm_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}
-
-size_t msm_lds_bytes(const MsmPlan& p) { return (size_t)p.NB * 4; }
-
-void msm_launch_sort(hzk_ctx* ctx, const MsmPlan& p, const void* scalars, uint32_t n, uint32_t* digits, uint32_t* hist,
-                     uint32_t* count, uint64_t* tiles, uint32_t ntiles, uint32_t* stats, uint32_t* offs,
-                     uint32_t* segbase) {
-    const uint32_t nk = (uint32_t)p.W * p.NB;
-    hipStream_t st = ctx->stream;
-    set_lds_limits();
-    stage_tables().digits[p.c](ctx, (const fr_t*)scalars, n, digits);
-    {
-        TimedLaunch tl(ctx, "msm_hist");
-        hipLaunchKernelGGL(k_msm_hist, dim3(p.T, p.W), dim3(1024), p.NB * 4, st, digits, n, p.tile, p.NB, nk, hist);
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, size_t n, int force_c) {
+    MsmPlan p = msm_plan(n, force_c ? force_c : ctx->force_window, ctx->force_seg, ctx->lds_per_block);
+    if (!force_c && !ctx->force_window) {
+        int cls = 0;
+        while (((size_t)1 << (cls + 1)) <= n) ++cls;
+        const hzk_tune_entry& t = ctx->tune[cls < 32 ? cls : 31];
+        if (t.c) {
+            MsmPlan q = msm_plan(n, t.c, t.g ? t.g : ctx->force_seg, ctx->lds_per_block);
+            if (t.K) q.K = t.K;
+            p = q;
+        }
     }
-    {
-        TimedLaunch tl(ctx, "msm_scan");
-        hipLaunchKernelGGL(k_msm_colscan, dim3(blocks_for(nk, 256)), dim3(256), 0, st, hist, nk, p.T, count);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, p.K, tiles, stats);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, ntiles, stats);
-        hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(SCAN_THREADS), 0, st, count, nk, p.K, tiles, offs, segbase);
-    }
+    return p;
 }
-
-void msm_launch_scatter(hzk_ctx* ctx, const MsmPlan& p, const uint32_t* digits, uint32_t n, const uint32_t* hist,
-                        const uint32_t* offs, uint32_t* sorted) {
-    const uint32_t nk = (uint32_t)p.W * p.NB;
-    TimedLaunch tl(ctx, "msm_scatter");
-    hipLaunchKernelGGL(k_msm_scatter, dim3(p.T, p.W), dim3(1024), p.NB * 4, ctx->stream, digits, n, p.tile, p.NB, nk, hist,
-                       offs, sorted);
-}
-
-uint32_t msm_scan_tiles(uint32_t nk) { return (nk + SCAN_TILE) / SCAN_TILE; }   // covers index nk (the total)
 
 }  // namespace hzk

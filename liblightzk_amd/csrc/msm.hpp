@@ -1,4 +1,5 @@
-// MSM entry points shared by the C ABI and the prover.
+// MSM plan, batch descriptors and entry points shared by the C ABI, the
+// batcher and the prover.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -19,19 +20,71 @@ struct MsmPlan {
     uint32_t T;     // scalar tiles per window (histogram / scatter workgroups)
     uint32_t tile;  // scalars per tile
 };
+// Plan for n points: autotuned entry for the size class if one is loaded,
+// else the built-in heuristic; force_c / force_g override (0 = automatic).
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes);
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, size_t n, int force_c);
 
-// scalar-side stages (msm.hip), shared by both groups: digits, LDS
-// histograms, scans (stats[0] = entries, [1] = segments, [2] = largest bucket)
-void msm_launch_sort(hzk_ctx* ctx, const MsmPlan& p, const void* scalars, uint32_t n, uint32_t* digits, uint32_t* hist,
-                     uint32_t* count, uint64_t* tiles, uint32_t ntiles, uint32_t* stats, uint32_t* offs,
-                     uint32_t* segbase);
-void msm_launch_scatter(hzk_ctx* ctx, const MsmPlan& p, const uint32_t* digits, uint32_t n, const uint32_t* hist,
-                        const uint32_t* offs, uint32_t* sorted);
-uint32_t msm_scan_tiles(uint32_t nk);
-size_t msm_lds_bytes(const MsmPlan& p);
+// Work spaces of the batched stages; each instance owns a contiguous range
+// [begin[s][i], begin[s][i+1]) of every space.
+enum MsmSpace {
+    SP_DIGIT = 0,   // one lane per scalar
+    SP_TILEWG,      // one workgroup per (tile, window)
+    SP_KEY,         // nk + 1 entries per instance (counts, offsets, segment bases)
+    SP_SCANTILE,    // scan tiles over the key space
+    SP_SEG,         // accumulation segments (upper bound)
+    SP_BRED,        // bucket-reduction lanes (window x run)
+    SP_WIN,         // windows
+    SP_ENTRY,       // n * W digit / sorted-entry slots
+    SP_HIST,        // T * nk per-tile histogram counters
+    SP_COUNT
+};
+
+// Device-side descriptor of one MSM instance of a batch.
+struct MsmInst {
+    const void* scalars;   // Fr[n], Montgomery
+    const void* bases;     // affine points
+    uint32_t n, c, W, NB, nk, K, g, G, T, tile;
+};
+
+struct MsmBatchDev {
+    const MsmInst* inst;      // [ninst]
+    const uint64_t* begin;    // [SP_COUNT][ninst + 1]
+    uint32_t ninst;
+    uint32_t* stats;          // per instance {entries, segments, largest bucket, 0}
+};
+
+// Launch the scalar-side stages (digits, LDS histograms, scans, scatter) for a
+// whole batch. Buffers are indexed through the work spaces above.
+struct MsmScalarBufs {
+    uint32_t* digits;    // SP_ENTRY
+    uint32_t* hist;      // SP_HIST
+    uint32_t* count;     // SP_KEY
+    uint32_t* offs;      // SP_KEY
+    uint32_t* segbase;   // SP_KEY
+    uint64_t* tiles;     // 2 * SP_SCANTILE
+    uint32_t* sorted;    // SP_ENTRY
+};
+void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t* host_totals, uint32_t max_nb,
+                              const MsmScalarBufs& bufs);
+constexpr uint32_t MSM_SCAN_TILE = 1024;
+// one-time kernel attribute setup (call outside graph capture)
+void msm_prepare();
 
 int msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest);
 int msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest);
+
+// Batched MSM: out[i] (device or host pointer per dests_on_host) receives the
+// normalised Jacobian result of instance i. capture != 0 records the device
+// stages into a hipGraph on first use of this batch shape and replays it.
+struct MsmJob {
+    const void* scalars;
+    const void* bases;
+    size_t n;
+    void* dest;
+    uint32_t win_siz;
+};
+int msm_batch_g1(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
+int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
 
 }  // namespace hzk

@@ -5,4 +5,7 @@ namespace hzk {
 int msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest) {
     return msm_run<fq2_t, host::G2x>(ctx, scalars, bases, n, win_siz, dest);
 }
+int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture) {
+    return msm_batch_run<fq2_t, host::G2x>(ctx, jobs, count, dests_on_host, capture);
+}
 }  // namespace hzk

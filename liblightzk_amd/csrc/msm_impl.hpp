@@ -1,8 +1,11 @@
-// Group-generic MSM kernels and driver (G1 over Fq, G2 over Fq2); included
-// by msm_g1.hip and msm_g2.hip, one instantiation per translation unit. The
-// stage list is in msm.hip.
+// Group-generic MSM stages 5-8 and the batched driver (G1 over Fq, G2 over
+// Fq2); included by msm_g1.hip and msm_g2.hip, one instantiation per
+// translation unit. Stages 1-4 and the batch layout are described in msm.hip.
 #pragma once
+#include <omp.h>
 #include <cstring>
+#include <map>
+#include <vector>
 #include "common.hpp"
 #include "curve.hpp"
 #include "host_ec.hpp"
@@ -12,8 +15,17 @@ namespace hzk {
 
 static constexpr int K2 = 16;   // merge fan-in
 
-// ------------------------------------------------------------------ accumulate
-// index of the key whose [segbase[k], segbase[k+1]) contains s
+HZK_DEV uint32_t find_inst_g(const uint64_t* __restrict__ beg, uint32_t ninst, uint64_t gid) {
+    uint32_t lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (beg[mid] <= gid) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+HZK_DEV const uint64_t* space_g(const MsmBatchDev& b, int s) { return b.begin + (size_t)s * (b.ninst + 1); }
+
+// index of the key whose [segbase[k], segbase[k+1]) contains s (keys 0..nk-1)
 HZK_DEV uint32_t find_key(const uint32_t* __restrict__ segbase, uint32_t nk, uint32_t s) {
     uint32_t lo = 0, hi = nk;   // invariant: segbase[lo] <= s < segbase[hi]
     while (hi - lo > 1) {
@@ -23,90 +35,118 @@ HZK_DEV uint32_t find_key(const uint32_t* __restrict__ segbase, uint32_t nk, uin
     return lo;
 }
 
+// ------------------------------------------------------------------ 5. accumulate
+// lane per segment of <= K entries of one bucket: mixed additions of affine
+// bases (negated for negative digits) into an XYZZ accumulator. Persistent:
+// a fixed grid strides over the whole batch's segment space.
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_accumulate(const affine_t<F>* __restrict__ bases, const uint32_t* __restrict__ sorted,
+__global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uint32_t* __restrict__ sorted,
                                                         const uint32_t* __restrict__ offs, const uint32_t* __restrict__ segbase,
-                                                        uint32_t nk, uint32_t nseg, uint32_t K,
                                                         xyzz_t<F>* __restrict__ part, uint32_t* __restrict__ seg2key) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    uint32_t k = find_key(segbase, nk, s);
-    uint32_t e = offs[k] + (s - segbase[k]) * K;
-    uint32_t e1 = min(e + K, offs[k + 1]);
-    xyzz_t<F> acc = xyzz_t<F>::inf();
-    uint32_t idx = sorted[e];
-    affine_t<F> P = bases[idx & 0x7fffffffu];
-    for (; e < e1; ++e) {
-        bool neg = idx >> 31;
-        affine_t<F> Q = P;
-        if (e + 1 < e1) {   // prefetch the next base while this one is added
-            idx = sorted[e + 1];
-            P = bases[idx & 0x7fffffffu];
+    const uint64_t* sbeg = space_g(b, SP_SEG);
+    const uint64_t total = sbeg[b.ninst];
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t ii = find_inst_g(sbeg, b.ninst, gid);
+        uint32_t s = (uint32_t)(gid - sbeg[ii]);
+        if (s >= b.stats[4 * ii + 1]) continue;
+        const MsmInst& d = b.inst[ii];
+        const uint64_t kb = space_g(b, SP_KEY)[ii];
+        const uint32_t* sb = segbase + kb;
+        const uint32_t* of = offs + kb;
+        uint32_t k = find_key(sb, d.nk, s);
+        uint32_t e = of[k] + (s - sb[k]) * d.K;
+        uint32_t e1 = min(e + d.K, of[k + 1]);
+        const uint32_t* srt = sorted + space_g(b, SP_ENTRY)[ii];
+        const affine_t<F>* bases = (const affine_t<F>*)d.bases;
+        xyzz_t<F> acc = xyzz_t<F>::inf();
+        uint32_t idx = srt[e];
+        affine_t<F> P = bases[idx & 0x7fffffffu];
+        for (; e < e1; ++e) {
+            bool neg = idx >> 31;
+            affine_t<F> Q = P;
+            if (e + 1 < e1) {   // prefetch the next base while this one is added
+                idx = srt[e + 1];
+                P = bases[idx & 0x7fffffffu];
+            }
+            if (neg) Q.y = Q.y.neg();
+            acc.add_affine(Q);
         }
-        if (neg) Q.y = Q.y.neg();
-        acc.add_affine(Q);
+        part[gid] = acc;
+        seg2key[gid] = k;
     }
-    part[s] = acc;
-    seg2key[s] = k;
 }
 
+// ------------------------------------------------------------------ 6. merge
 // keys with 2..K2 segments: one lane per key sums them into the first slot
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_merge_small(const uint32_t* __restrict__ segbase, uint32_t nk,
+__global__ void __launch_bounds__(256) k_msm_merge_small(MsmBatchDev b, const uint32_t* __restrict__ segbase,
                                                          xyzz_t<F>* __restrict__ part) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nk) return;
-    uint32_t b0 = segbase[k], cnt = segbase[k + 1] - b0;
+    const uint64_t* kbeg = space_g(b, SP_KEY);
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= kbeg[b.ninst]) return;
+    uint32_t ii = find_inst_g(kbeg, b.ninst, gid);
+    uint32_t k = (uint32_t)(gid - kbeg[ii]);
+    if (k >= b.inst[ii].nk) return;
+    uint32_t b0 = segbase[gid], cnt = segbase[gid + 1] - b0;
     if (cnt < 2 || cnt > K2) return;
-    xyzz_t<F> acc = part[b0];
-    for (uint32_t i = 1; i < cnt; ++i) acc.add(part[b0 + i]);
-    part[b0] = acc;
+    xyzz_t<F>* p = part + space_g(b, SP_SEG)[ii] + b0;
+    xyzz_t<F> acc = p[0];
+    for (uint32_t i = 1; i < cnt; ++i) acc.add(p[i]);
+    p[0] = acc;
 }
 
 // keys with more than K2 segments (oversized buckets): one level of an
 // in-place K2-ary tree over the key's segment sums
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_merge(const uint32_t* __restrict__ segbase, const uint32_t* __restrict__ seg2key,
-                                                   uint32_t nseg, uint32_t stride, xyzz_t<F>* __restrict__ part) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    uint32_t k = seg2key[s];
-    uint32_t b0 = segbase[k], cnt = segbase[k + 1] - b0, rel = s - b0;
+__global__ void __launch_bounds__(256) k_msm_merge(MsmBatchDev b, const uint32_t* __restrict__ segbase,
+                                                   const uint32_t* __restrict__ seg2key, uint32_t stride,
+                                                   xyzz_t<F>* __restrict__ part) {
+    const uint64_t* sbeg = space_g(b, SP_SEG);
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= sbeg[b.ninst]) return;
+    uint32_t ii = find_inst_g(sbeg, b.ninst, gid);
+    uint32_t s = (uint32_t)(gid - sbeg[ii]);
+    if (s >= b.stats[4 * ii + 1]) return;
+    const uint32_t* sb = segbase + space_g(b, SP_KEY)[ii];
+    uint32_t k = seg2key[gid];
+    uint32_t b0 = sb[k], cnt = sb[k + 1] - b0, rel = s - b0;
     if (cnt <= K2 || cnt <= stride || rel % (stride * K2) != 0) return;
-    xyzz_t<F> acc = part[s];
+    xyzz_t<F>* p = part + sbeg[ii] + b0;
+    xyzz_t<F> acc = p[rel];
     for (int i = 1; i < K2; ++i) {
         uint32_t r = rel + i * stride;
         if (r >= cnt) break;
-        acc.add(part[b0 + r]);
+        acc.add(p[r]);
     }
-    part[s] = acc;
+    p[rel] = acc;
 }
 
-// ------------------------------------------------------------------ bucket reduction
-template <class F>
-HZK_DEV xyzz_t<F> bucket_sum(const xyzz_t<F>* __restrict__ part, const uint32_t* __restrict__ offs,
-                              const uint32_t* __restrict__ segbase, uint32_t key) {
-    if (offs[key + 1] == offs[key]) return xyzz_t<F>::inf();
-    return part[segbase[key]];
-}
-
+// ------------------------------------------------------------------ 7. bucket reduction
 // lane per (window, run of g buckets): T = sum_{d in run} d * B_d
 template <class F>
-__global__ void __launch_bounds__(128) k_bucket_segments(const xyzz_t<F>* __restrict__ part, const uint32_t* __restrict__ offs,
-                                                         const uint32_t* __restrict__ segbase, uint32_t W, uint32_t NB, uint32_t g,
+__global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xyzz_t<F>* __restrict__ part,
+                                                         const uint32_t* __restrict__ offs, const uint32_t* __restrict__ segbase,
                                                          xyzz_t<F>* __restrict__ segsum) {
-    uint32_t G = NB / g;
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= W * G) return;
-    uint32_t j = t / G, sg = t % G;
-    uint32_t kb = j * NB + sg * g;
+    const uint64_t* rbeg = space_g(b, SP_BRED);
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= rbeg[b.ninst]) return;
+    uint32_t ii = find_inst_g(rbeg, b.ninst, gid);
+    const MsmInst& d = b.inst[ii];
+    uint32_t t = (uint32_t)(gid - rbeg[ii]);
+    uint32_t j = t / d.G, sg = t % d.G;
+    const uint64_t kb = space_g(b, SP_KEY)[ii];
+    const uint32_t* of = offs + kb;
+    const uint32_t* sb = segbase + kb;
+    const xyzz_t<F>* pp = part + space_g(b, SP_SEG)[ii];
+    uint32_t kbase = j * d.NB + sg * d.g;
     xyzz_t<F> run = xyzz_t<F>::inf(), acc = xyzz_t<F>::inf();
-    for (int b = (int)g - 1; b >= 0; --b) {
-        run.add(bucket_sum<F>(part, offs, segbase, kb + b));
+    for (int q = (int)d.g - 1; q >= 0; --q) {
+        uint32_t key = kbase + q;
+        if (of[key + 1] != of[key]) run.add(pp[sb[key]]);
         acc.add(run);
     }
     // acc = sum (d - sg*g) B_d ; add (sg*g) * run
-    uint32_t w = sg * g;
+    uint32_t w = sg * d.g;
     if (w) {
         xyzz_t<F> m = xyzz_t<F>::inf();
         for (int bit = 31 - __clz(w); bit >= 0; --bit) {
@@ -115,16 +155,21 @@ __global__ void __launch_bounds__(128) k_bucket_segments(const xyzz_t<F>* __rest
         }
         acc.add(m);
     }
-    segsum[t] = acc;
+    segsum[gid] = acc;
 }
 
-// block per window: tree sum of the window's segment sums
+// block per window: tree sum of the window's run sums
 template <class F>
-__global__ void __launch_bounds__(256) k_window_sum(const xyzz_t<F>* __restrict__ segsum, uint32_t G, xyzz_t<F>* __restrict__ winsum) {
+__global__ void __launch_bounds__(256) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
+                                                    xyzz_t<F>* __restrict__ winsum) {
     __shared__ xyzz_t<F> sh[256];
-    uint32_t j = blockIdx.x;
+    const uint64_t* wbeg = space_g(b, SP_WIN);
+    uint32_t ii = find_inst_g(wbeg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    uint32_t j = (uint32_t)(blockIdx.x - wbeg[ii]);
+    const xyzz_t<F>* ss = segsum + space_g(b, SP_BRED)[ii] + (size_t)j * d.G;
     xyzz_t<F> acc = xyzz_t<F>::inf();
-    for (uint32_t s = threadIdx.x; s < G; s += 256) acc.add(segsum[j * G + s]);
+    for (uint32_t s = threadIdx.x; s < d.G; s += 256) acc.add(ss[s]);
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -135,109 +180,234 @@ __global__ void __launch_bounds__(256) k_window_sum(const xyzz_t<F>* __restrict_
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) winsum[j] = sh[0];
+    if (threadIdx.x == 0) winsum[blockIdx.x] = sh[0];
 }
 
 // ------------------------------------------------------------------ driver
-template <class F, class HostPt>
-int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest) {
-    HZK_CHECK(ctx, n > 0 && n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", n);
-    HZK_CHECK(ctx, scalars && bases && dest, HZK_ERR_INVALID, "msm: null pointer");
-    HZK_CHECK(ctx, win_siz == 0 || (win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C), HZK_ERR_INVALID,
-              "msm: win_siz=%zu unsupported (0 or %d..%d)", win_siz, MSM_MIN_C, MSM_MAX_C);
-    MsmPlan p = msm_plan(n, win_siz ? (int)win_siz : ctx->force_window, ctx->force_seg, ctx->lds_per_block);
-    HZK_CHECK(ctx, msm_lds_bytes(p) <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
-              "msm: window %d needs %zu B of LDS per workgroup, device allows %zu", p.c, msm_lds_bytes(p), ctx->lds_per_block);
-    HZK_CHECK(ctx, p.g >= 1 && (p.g & (p.g - 1)) == 0 && p.g <= p.NB, HZK_ERR_INVALID, "msm: bad segment size %u", p.g);
-    const uint32_t nk = (uint32_t)p.W * p.NB;
-    const size_t Lmax = n * (size_t)p.W;
-    const size_t Smax = Lmax / p.K + nk;
-    const uint32_t ntiles = msm_scan_tiles(nk);
-    const uint32_t G = p.NB / p.g;
+struct MsmBatchLayout {
+    std::vector<MsmInst> inst;
+    std::vector<uint64_t> begin;   // [SP_COUNT][ninst + 1]
+    uint64_t tot[SP_COUNT];
+    uint32_t max_nb = 1;
+    uint32_t merge_levels = 0;     // tree levels for oversized buckets (upper bound)
+    bool any_merge = false;
+    size_t bytes = 0;              // scratch needed
+    // scratch offsets
+    size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_segbase, o_tiles, o_sorted, o_seg2key, o_part,
+        o_segsum, o_win;
+};
 
-    size_t need = 0;
-    auto add = [&need](size_t b) { need = ((need + 255) & ~size_t(255)) + b; };
-    add(Lmax * 4ull);                           // digits (window-major)
-    add((size_t)p.T * nk * 4ull);               // per-tile histograms
-    add(nk * 4ull);                             // count
-    add((nk + 1) * 4ull);                       // offs
-    add((nk + 1) * 4ull);                       // segbase
-    add(2ull * ntiles * 8ull);                  // scan tile sums + maxima
-    add(64);                                    // stats
-    add(Lmax * 4ull);                           // sorted
-    add(Smax * 4ull);                           // seg2key
-    add(Smax * sizeof(xyzz_t<F>));              // part
-    add((size_t)p.W * G * sizeof(xyzz_t<F>));   // segsum
-    add((size_t)p.W * sizeof(xyzz_t<F>));       // winsum
-    int rc = ensure_scratch(ctx, need + 4096);
-    if (rc) return rc;
-    rc = ensure_pinned(ctx, 64 + p.W * sizeof(xyzz_t<F>));
-    if (rc) return rc;
-    Arena ar{(char*)ctx->scratch, ctx->scratch_bytes};
-    uint32_t* digits = (uint32_t*)ar.take(Lmax * 4ull);
-    uint32_t* hist = (uint32_t*)ar.take((size_t)p.T * nk * 4ull);
-    uint32_t* count = (uint32_t*)ar.take(nk * 4ull);
-    uint32_t* offs = (uint32_t*)ar.take((nk + 1) * 4ull);
-    uint32_t* segbase = (uint32_t*)ar.take((nk + 1) * 4ull);
-    uint64_t* tiles = (uint64_t*)ar.take(2ull * ntiles * 8ull);
-    uint32_t* stats = (uint32_t*)ar.take(64);
-    uint32_t* sorted = (uint32_t*)ar.take(Lmax * 4ull);
-    uint32_t* seg2key = (uint32_t*)ar.take(Smax * 4ull);
-    xyzz_t<F>* part = (xyzz_t<F>*)ar.take(Smax * sizeof(xyzz_t<F>));
-    xyzz_t<F>* segsum = (xyzz_t<F>*)ar.take((size_t)p.W * G * sizeof(xyzz_t<F>));
-    xyzz_t<F>* winsum = (xyzz_t<F>*)ar.take((size_t)p.W * sizeof(xyzz_t<F>));
+template <class F>
+static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count) {
+    MsmBatchLayout L;
+    L.inst.resize(count);
+    L.begin.assign((size_t)SP_COUNT * (count + 1), 0);
+    for (size_t i = 0; i < count; ++i) {
+        MsmPlan p = msm_plan_ctx(ctx, jobs[i].n, (int)jobs[i].win_siz);
+        MsmInst& d = L.inst[i];
+        d.scalars = jobs[i].scalars;
+        d.bases = jobs[i].bases;
+        d.n = (uint32_t)jobs[i].n;
+        d.c = (uint32_t)p.c;
+        d.W = (uint32_t)p.W;
+        d.NB = p.NB;
+        d.nk = d.W * d.NB;
+        d.K = p.K;
+        d.g = p.g;
+        d.G = p.NB / p.g;
+        d.T = p.T;
+        d.tile = p.tile;
+        uint64_t entries = (uint64_t)d.n * d.W;
+        uint64_t sz[SP_COUNT];
+        sz[SP_DIGIT] = d.n;
+        sz[SP_TILEWG] = (uint64_t)d.T * d.W;
+        sz[SP_KEY] = (uint64_t)d.nk + 1;
+        sz[SP_SCANTILE] = ((uint64_t)d.nk + MSM_SCAN_TILE) / MSM_SCAN_TILE;
+        sz[SP_SEG] = (entries + d.K - 1) / d.K + d.nk;
+        sz[SP_BRED] = (uint64_t)d.W * d.G;
+        sz[SP_WIN] = d.W;
+        sz[SP_ENTRY] = entries;
+        sz[SP_HIST] = (uint64_t)d.T * d.nk;
+        for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
+        if (d.NB > L.max_nb) L.max_nb = d.NB;
+        uint32_t maxseg = (d.n + d.K - 1) / d.K;   // a bucket holds at most n entries
+        if (maxseg > 1) L.any_merge = true;
+        uint32_t lv = 0;
+        if (maxseg > (uint32_t)K2)
+            for (uint64_t stride = 1; stride < maxseg; stride *= K2) ++lv;
+        if (lv > L.merge_levels) L.merge_levels = lv;
+    }
+    for (int s = 0; s < SP_COUNT; ++s) L.tot[s] = L.begin[(size_t)s * (count + 1) + count];
+    size_t off = 0;
+    auto take = [&off](size_t bytes) { size_t o = (off + 255) & ~size_t(255); off = o + bytes; return o; };
+    L.o_inst = take(count * sizeof(MsmInst));
+    L.o_begin = take(L.begin.size() * 8);
+    L.o_stats = take(count * 16);
+    L.o_digits = take(L.tot[SP_ENTRY] * 4);
+    L.o_hist = take(L.tot[SP_HIST] * 4);
+    L.o_count = take(L.tot[SP_KEY] * 4);
+    L.o_offs = take(L.tot[SP_KEY] * 4);
+    L.o_segbase = take(L.tot[SP_KEY] * 4);
+    L.o_tiles = take(2 * L.tot[SP_SCANTILE] * 8);
+    L.o_sorted = take(L.tot[SP_ENTRY] * 4);
+    L.o_seg2key = take(L.tot[SP_SEG] * 4);
+    L.o_part = take(L.tot[SP_SEG] * sizeof(xyzz_t<F>));
+    L.o_segsum = take(L.tot[SP_BRED] * sizeof(xyzz_t<F>));
+    L.o_win = take(L.tot[SP_WIN] * sizeof(xyzz_t<F>));
+    L.bytes = off + 256;
+    return L;
+}
 
+// device stages 1-8 for a laid-out batch (all launches on ctx->stream; no host waits)
+template <class F>
+static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base) {
+    const size_t ninst = L.inst.size();
+    MsmBatchDev b;
+    b.inst = (const MsmInst*)(base + L.o_inst);
+    b.begin = (const uint64_t*)(base + L.o_begin);
+    b.ninst = (uint32_t)ninst;
+    b.stats = (uint32_t*)(base + L.o_stats);
+    MsmScalarBufs sb;
+    sb.digits = (uint32_t*)(base + L.o_digits);
+    sb.hist = (uint32_t*)(base + L.o_hist);
+    sb.count = (uint32_t*)(base + L.o_count);
+    sb.offs = (uint32_t*)(base + L.o_offs);
+    sb.segbase = (uint32_t*)(base + L.o_segbase);
+    sb.tiles = (uint64_t*)(base + L.o_tiles);
+    sb.sorted = (uint32_t*)(base + L.o_sorted);
+    uint32_t* seg2key = (uint32_t*)(base + L.o_seg2key);
+    xyzz_t<F>* part = (xyzz_t<F>*)(base + L.o_part);
+    xyzz_t<F>* segsum = (xyzz_t<F>*)(base + L.o_segsum);
+    xyzz_t<F>* win = (xyzz_t<F>*)(base + L.o_win);
     hipStream_t st = ctx->stream;
-    msm_launch_sort(ctx, p, scalars, (uint32_t)n, digits, hist, count, tiles, ntiles, stats, offs, segbase);
-    HZK_HIP(ctx, hipGetLastError());
-    uint32_t* hstats = (uint32_t*)ctx->pinned;
-    HZK_HIP(ctx, hipMemcpyAsync(hstats, stats, 16, hipMemcpyDeviceToHost, st));
-    msm_launch_scatter(ctx, p, digits, (uint32_t)n, hist, offs, sorted);
-    HZK_HIP(ctx, hipGetLastError());
-    HZK_HIP(ctx, hipStreamSynchronize(st));
-    const uint32_t L = hstats[0], S = hstats[1], maxc = hstats[2];
-    HZK_CHECK(ctx, L <= Lmax && S <= Smax, HZK_ERR_INVALID, "msm: internal size check failed (L=%u S=%u)", L, S);
-    ctx->last_msm[0] = L; ctx->last_msm[1] = S; ctx->last_msm[2] = maxc;
-    ctx->last_msm[3] = (uint32_t)p.c; ctx->last_msm[4] = (uint32_t)p.W;
 
-    if (S > 0) {
-        {
-            TimedLaunch tl(ctx, "msm_accumulate");
-            hipLaunchKernelGGL(k_msm_accumulate<F>, dim3(blocks_for(S, 256)), dim3(256), 0, st, (const affine_t<F>*)bases,
-                               sorted, offs, segbase, nk, S, p.K, part, seg2key);
-        }
-        uint32_t maxseg = (maxc + p.K - 1) / p.K;
-        if (maxseg > 1) {
-            TimedLaunch tl(ctx, "msm_merge");
-            hipLaunchKernelGGL(k_msm_merge_small<F>, dim3(blocks_for(nk, 256)), dim3(256), 0, st, segbase, nk, part);
-        }
-        for (uint32_t stride = 1; maxseg > (uint32_t)K2 && stride < maxseg; stride *= K2) {
-            TimedLaunch tl(ctx, "msm_merge");
-            hipLaunchKernelGGL(k_msm_merge<F>, dim3(blocks_for(S, 256)), dim3(256), 0, st, segbase, seg2key, S, stride, part);
-        }
+    msm_launch_scalar_stages(ctx, b, L.tot, L.max_nb, sb);
+    {
+        TimedLaunch tl(ctx, "msm_accumulate");
+        uint64_t blocks = (L.tot[SP_SEG] + 255) / 256;
+        uint64_t cap = (uint64_t)ctx->num_cus * 16;   // persistent grid: 16 waves per CU
+        hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
+                           (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, (const uint32_t*)sb.segbase, part, seg2key);
+    }
+    if (L.any_merge) {
+        TimedLaunch tl(ctx, "msm_merge");
+        hipLaunchKernelGGL(k_msm_merge_small<F>, dim3(blocks_for(L.tot[SP_KEY], 256)), dim3(256), 0, st, b,
+                           (const uint32_t*)sb.segbase, part);
+        uint32_t stride = 1;
+        for (uint32_t lv = 0; lv < L.merge_levels; ++lv, stride *= K2)
+            hipLaunchKernelGGL(k_msm_merge<F>, dim3(blocks_for(L.tot[SP_SEG], 256)), dim3(256), 0, st, b,
+                               (const uint32_t*)sb.segbase, (const uint32_t*)seg2key, stride, part);
     }
     {
         TimedLaunch tl(ctx, "msm_bucket_reduce");
-        hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for((size_t)p.W * G, 128)), dim3(128), 0, st,
-                           (const xyzz_t<F>*)part, offs, segbase, (uint32_t)p.W, p.NB, p.g, segsum);
-        hipLaunchKernelGGL(k_window_sum<F>, dim3(p.W), dim3(256), 0, st, (const xyzz_t<F>*)segsum, G, winsum);
+        hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b,
+                           (const xyzz_t<F>*)part, (const uint32_t*)sb.offs, (const uint32_t*)sb.segbase, segsum);
+        hipLaunchKernelGGL(k_window_sum<F>, dim3((unsigned)L.tot[SP_WIN]), dim3(256), 0, st, b, (const xyzz_t<F>*)segsum, win);
+    }
+}
+
+template <class F>
+static std::string shape_key(const MsmBatchLayout& L) {
+    std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
+    for (const MsmInst& d : L.inst) {
+        char buf[64];
+        snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T);
+        k += buf;
+    }
+    return k;
+}
+
+template <class F, class HostPt>
+int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture) {
+    HZK_CHECK(ctx, count > 0 && jobs, HZK_ERR_INVALID, "msm: empty batch");
+    for (size_t i = 0; i < count; ++i) {
+        HZK_CHECK(ctx, jobs[i].n > 0 && jobs[i].n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", jobs[i].n);
+        HZK_CHECK(ctx, jobs[i].scalars && jobs[i].bases && jobs[i].dest, HZK_ERR_INVALID, "msm: null pointer in job %zu", i);
+        HZK_CHECK(ctx, jobs[i].win_siz == 0 || (jobs[i].win_siz >= (uint32_t)MSM_MIN_C && jobs[i].win_siz <= (uint32_t)MSM_MAX_C),
+                  HZK_ERR_INVALID, "msm: win_siz=%u unsupported (0 or %d..%d)", jobs[i].win_siz, MSM_MIN_C, MSM_MAX_C);
+    }
+    msm_prepare();
+    MsmBatchLayout L = msm_layout<F>(ctx, jobs, count);
+    HZK_CHECK(ctx, (size_t)L.max_nb * 4 <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
+              "msm: a %u-bucket window needs %u B of LDS per workgroup, device allows %zu", L.max_nb, L.max_nb * 4,
+              ctx->lds_per_block);
+    int rc = ensure_scratch(ctx, L.bytes);
+    if (rc) return rc;
+    const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 + 1024;
+    rc = ensure_pinned(ctx, host_bytes);
+    if (rc) return rc;
+    char* base = (char*)ctx->scratch;
+    hipStream_t st = ctx->stream;
+    // descriptors and work-space starts
+    char* pin = (char*)ctx->pinned;
+    std::memcpy(pin, L.inst.data(), count * sizeof(MsmInst));
+    std::memcpy(pin + count * sizeof(MsmInst), L.begin.data(), L.begin.size() * 8);
+    HZK_HIP(ctx, hipMemcpyAsync(base + L.o_inst, pin, count * sizeof(MsmInst), hipMemcpyHostToDevice, st));
+    HZK_HIP(ctx, hipMemcpyAsync(base + L.o_begin, pin + count * sizeof(MsmInst), L.begin.size() * 8, hipMemcpyHostToDevice, st));
+
+    if (capture) {
+        std::string key = shape_key<F>(L);
+        hzk_graph_entry& g = ctx->graphs[key];
+        if (!g.exec || g.generation != ctx->scratch_generation) {
+            if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+            hipGraph_t graph;
+            bool timing = ctx->timing;
+            ctx->timing = false;
+            HZK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            msm_device_stages<F>(ctx, L, base);
+            hipError_t e = hipStreamEndCapture(st, &graph);
+            ctx->timing = timing;
+            HZK_HIP(ctx, e);
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HZK_HIP(ctx, e);
+            g.generation = ctx->scratch_generation;
+        }
+        TimedLaunch tl(ctx, "msm_graph");
+        HZK_HIP(ctx, hipGraphLaunch(g.exec, st));
+    } else {
+        msm_device_stages<F>(ctx, L, base);
     }
     HZK_HIP(ctx, hipGetLastError());
-    char* hwin = (char*)ctx->pinned + 64;
-    HZK_HIP(ctx, hipMemcpyAsync(hwin, winsum, p.W * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
+    char* hwin = pin + count * sizeof(MsmInst) + L.begin.size() * 8;
+    uint32_t* hstats = (uint32_t*)(hwin + L.tot[SP_WIN] * sizeof(xyzz_t<F>));
+    HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
+    HZK_HIP(ctx, hipMemcpyAsync(hstats, base + L.o_stats, count * 16, hipMemcpyDeviceToHost, st));
     HZK_HIP(ctx, hipStreamSynchronize(st));
 
-    // Horner over windows on the host
-    HostPt acc = HostPt::inf();
-    for (int j = p.W - 1; j >= 0; --j) {
-        for (int b = 0; b < p.c; ++b) acc = acc.dbl();
-        acc = acc.add(HostPt::load_xyzz(hwin + j * sizeof(xyzz_t<F>)));
+    // window combine (Horner over windows) on the host, instances in parallel
+    const uint64_t* wbeg = L.begin.data() + (size_t)SP_WIN * (count + 1);
+    std::vector<char> out(count * 3 * sizeof(F));
+#pragma omp parallel for schedule(dynamic, 1) if (count > 1)
+    for (long i = 0; i < (long)count; ++i) {
+        const MsmInst& d = L.inst[i];
+        HostPt acc = HostPt::inf();
+        for (int j = (int)d.W - 1; j >= 0; --j) {
+            for (uint32_t q = 0; q < d.c; ++q) acc = acc.dbl();
+            acc = acc.add(HostPt::load_xyzz(hwin + (wbeg[i] + j) * sizeof(xyzz_t<F>)));
+        }
+        acc.store_jacobian(&out[i * 3 * sizeof(F)]);
     }
-    char out[192];
-    acc.store_jacobian(out);
-    HZK_HIP(ctx, hipMemcpyAsync(dest, out, 3 * sizeof(F), hipMemcpyHostToDevice, st));
-    HZK_HIP(ctx, hipStreamSynchronize(st));
+    for (size_t i = 0; i < count; ++i) {
+        if (dests_on_host) std::memcpy(jobs[i].dest, &out[i * 3 * sizeof(F)], 3 * sizeof(F));
+        else HZK_HIP(ctx, hipMemcpyAsync(jobs[i].dest, &out[i * 3 * sizeof(F)], 3 * sizeof(F), hipMemcpyHostToDevice, st));
+    }
+    if (!dests_on_host) HZK_HIP(ctx, hipStreamSynchronize(st));
+    const MsmInst& d0 = L.inst[count - 1];
+    ctx->last_msm[0] = hstats[4 * (count - 1)];
+    ctx->last_msm[1] = hstats[4 * (count - 1) + 1];
+    ctx->last_msm[2] = hstats[4 * (count - 1) + 2];
+    ctx->last_msm[3] = d0.c;
+    ctx->last_msm[4] = d0.W;
     return 0;
+}
+
+template <class F, class HostPt>
+int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest) {
+    HZK_CHECK(ctx, win_siz <= (size_t)MSM_MAX_C, HZK_ERR_INVALID, "msm: win_siz=%zu unsupported (0 or %d..%d)", win_siz,
+              MSM_MIN_C, MSM_MAX_C);
+    MsmJob j{scalars, bases, n, dest, (uint32_t)win_siz};
+    return msm_batch_run<F, HostPt>(ctx, &j, 1, 0, 0);
 }
 
 }  // namespace hzk

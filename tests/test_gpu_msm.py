@@ -147,3 +147,56 @@ def test_fixbase_g1_g2(ctx, orc, n):
     assert np.array_equal(o1.download(np.uint64, 8 * n), orc.g1_mul_gen(k))
     assert np.array_equal(o2.download(np.uint64, 16 * n), orc.g2_mul_gen(k))
     assert np.array_equal(dk.download(np.uint64, 4 * n), k)   # scalars untouched
+
+
+def _batch_case(orc, seed, sizes, group):
+    tot = sum(sizes)
+    s = orc.gen_fr(seed, 0, tot)
+    k = orc.gen_fr(seed, 1, tot)
+    bases = orc.g1_mul_gen(k) if group == 1 else orc.g2_mul_gen(k)
+    return s, k, bases
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_g1_batch_uneven_sizes(ctx, orc, graph):
+    """Batched MSMs (one launch per stage over all instances, optionally one
+    hipGraph): every result equals the same instance run alone and the closed form."""
+    rng = np.random.default_rng(7)
+    sizes = [int(2 ** rng.uniform(4, 14)) for _ in range(24)] + [1, 2]
+    s, k, bases = _batch_case(orc, 21, sizes, 1)
+    ds, db = ctx.upload(s), ctx.upload(bases)
+    outs = ctx.alloc(96 * len(sizes))
+    for rep in range(2):   # second pass replays the captured graph with new scalars
+        if rep == 1:
+            s = orc.gen_fr(22, 0, sum(sizes))
+            ds.upload(s)
+        jobs, off = [], 0
+        for i, n in enumerate(sizes):
+            jobs.append((ds.offset(32 * off), db.offset(64 * off), n, outs.offset(96 * i)))
+            off += n
+        ctx.msm_batch(1, jobs, graph=graph)
+        got = outs.download(np.uint64, 12 * len(sizes))
+        off = 0
+        for i, n in enumerate(sizes):
+            want = orc.msm_closed_form(s[4 * off: 4 * (off + n)].copy(), k[4 * off: 4 * (off + n)].copy(), 1)
+            assert np.array_equal(got[12 * i: 12 * i + 12], want), (rep, i, n)
+            single = _run_g1(ctx, s[4 * off: 4 * (off + n)].copy(), bases[8 * off: 8 * (off + n)].copy())
+            assert np.array_equal(single, want), (rep, i, n)
+            off += n
+
+
+def test_g2_batch_host_dest(ctx, orc):
+    sizes = [5, 300, 1000, 64]
+    s, k, bases = _batch_case(orc, 23, sizes, 2)
+    ds, db = ctx.upload(s), ctx.upload(bases)
+    host_out = [np.zeros(24, dtype=np.uint64) for _ in sizes]
+    jobs, off = [], 0
+    for i, n in enumerate(sizes):
+        jobs.append((ds.offset(32 * off), db.offset(128 * off), n, host_out[i].ctypes.data))
+        off += n
+    ctx.msm_batch(2, jobs, host_dest=True)
+    off = 0
+    for i, n in enumerate(sizes):
+        want = orc.msm_closed_form(s[4 * off: 4 * (off + n)].copy(), k[4 * off: 4 * (off + n)].copy(), 2)
+        assert np.array_equal(host_out[i], want), (i, n)
+        off += n
