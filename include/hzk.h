@@ -35,6 +35,10 @@ extern "C" {
 
 typedef struct hzk_ctx hzk_ctx;
 
+/* batch flags */
+#define HZK_BATCH_HOST_DEST 1u   /* MSM results written to host pointers */
+#define HZK_BATCH_GRAPH 2u       /* capture the device stages into a hipGraph and replay it */
+
 /* ---- context (replaces the reference's implicit CUDA runtime state and legacy
  * default stream, include/device-arith/device-utils.cuh:3) ---- */
 int hzk_ctx_create(int device, void* hip_stream /* nullable: own stream */, hzk_ctx** out);
@@ -84,8 +88,6 @@ typedef struct hzk_msm_desc {
     void* dest;            /* Jacobian result (device, or host with HZK_BATCH_HOST_DEST) */
     uint32_t win_siz;      /* 0 = autotuned */
 } hzk_msm_desc;
-#define HZK_BATCH_HOST_DEST 1u
-#define HZK_BATCH_GRAPH 2u
 int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
 int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
 
@@ -124,6 +126,17 @@ int hzk_spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C
 /* A <- A o B - C over deg Fr elements (replaces polynomial_A_times_B_minus_C,
  * include/operator/poly.cuh:75-86) */
 int hzk_pointwise_abc(hzk_ctx* ctx, size_t deg, void* A, const void* B, const void* C);
+
+/* Batched transforms of uneven sizes (no reference counterpart). kind: 0 NTT,
+ * 1 inverse NTT (scaled by 1/deg), 2 coset NTT, 3 coset inverse NTT, all with
+ * the library's tables; in place. With HZK_BATCH_GRAPH the launches are
+ * captured into a hipGraph per batch shape and replayed. */
+typedef struct hzk_ntt_desc {
+    void* coeffs;
+    size_t deg;
+    int kind;
+} hzk_ntt_desc;
+int hzk_ntt_batch(hzk_ctx* ctx, const hzk_ntt_desc* d, size_t count, uint32_t flags);
 
 /* ---- fixed-base generation (replaces fix_base_multi_scalar_multiplication_g1/_g2,
  * include/operator/msm.cuh:5-46, plus the affine normalisation of pre_comp_g1/_g2,

@@ -36,6 +36,11 @@ class MsmDesc(ctypes.Structure):
                 ("dest", ctypes.c_void_p), ("win_siz", ctypes.c_uint32)]
 
 
+class NttDesc(ctypes.Structure):
+    """hzk_ntt_desc: one transform of a batch (kind 0 NTT, 1 iNTT, 2 coset NTT, 3 coset iNTT)."""
+    _fields_ = [("coeffs", ctypes.c_void_p), ("deg", ctypes.c_size_t), ("kind", ctypes.c_int)]
+
+
 class DeviceBuffer:
     """Owned device allocation (xpu::vector with mem_policy::device_only)."""
 
@@ -192,6 +197,13 @@ class Context:
         flags = (1 if host_dest else 0) | (2 if graph else 0)
         fn = load().hzk_msm_g1_batch if group == 1 else load().hzk_msm_g2_batch
         self._check(fn(self.handle, arr, len(jobs), flags))
+
+    def ntt_batch(self, jobs, graph=False):
+        """jobs: list of (device pointer, deg, kind)."""
+        arr = (NttDesc * len(jobs))()
+        for i, (p, deg, kind) in enumerate(jobs):
+            arr[i] = NttDesc(int(p), deg, kind)
+        self._check(load().hzk_ntt_batch(self.handle, arr, len(jobs), 2 if graph else 0))
 
     def msm_last_stats(self):
         a = (ctypes.c_uint32 * 5)()

@@ -168,6 +168,7 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     ntt_release(ctx);
+    if (ctx->desc) (void)hipFree(ctx->desc);
     if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);
     if (ctx->comb_g2) (void)hipFree(ctx->comb_g2);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);

@@ -31,6 +31,10 @@ struct hzk_ctx {
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
     uint64_t scratch_generation = 0;   // bumped on every reallocation (invalidates captured graphs)
+    // small device buffer for batch descriptors of graph-captured work (ntt.hip)
+    void* desc = nullptr;
+    size_t desc_bytes = 0;
+    uint64_t desc_generation = 0;
     // captured device-stage graphs keyed by batch shape
     std::map<std::string, hzk_graph_entry> graphs;
     // pinned host staging for small results
@@ -92,6 +96,12 @@ int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, co
 int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale);
 // in-place standard NTT (inverse: w^-1 and 1/n) and coset NTT over Fr (ntt.hip)
 int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
+struct NttJob {
+    void* x;
+    size_t deg;
+    int kind;   // 0 NTT, 1 inverse NTT, 2 coset NTT, 3 coset inverse NTT
+};
+int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture);
 int coset_ntt_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 
 // Kernel timing helpers (no-ops unless ctx->timing).

@@ -15,6 +15,8 @@
 // runs R stages there, and writes the tile back. 2^20 takes two passes
 // (R = 9, then the last 11 stages, whose rows are contiguous). One more pass
 // applies the bit-reverse permutation fused with the scale factor.
+#include <algorithm>
+#include <string>
 #include <vector>
 #include "common.hpp"
 #include "field.hpp"
@@ -26,15 +28,12 @@ static constexpr int NTT_TILE_BITS = 11;
 static constexpr int NTT_THREADS = 256;
 static constexpr int NTT_MAX_LOG = 26;
 
-// one pass of stages [s0, s0 + R) over tiles of 2^R rows x 2^gbits columns
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k, int s0,
-                                                          int R, int gbits) {
-    extern __shared__ fr_t sh[];
+// one tile of a pass of stages [s0, s0 + R) over tiles of 2^R rows x 2^gbits columns
+HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k, int s0, int R, int gbits, uint32_t b, fr_t* sh) {
     const int low = k - s0 - R;
     const uint32_t G = 1u << gbits;
     const uint32_t T = 1u << (R + gbits);
     const uint32_t lchunks = (1u << low) >> gbits;
-    const uint32_t b = blockIdx.x;
     const uint32_t hi = b / lchunks, lc = b % lchunks;
     const size_t base = ((size_t)hi << (k - s0)) | ((size_t)lc << gbits);
     for (uint32_t e = threadIdx.x; e < T; e += NTT_THREADS) {
@@ -61,6 +60,85 @@ __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, 
         uint32_t m = e >> gbits, g = e & (G - 1);
         x[base + ((size_t)m << low) + g] = sh[e];
     }
+}
+
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k, int s0,
+                                                          int R, int gbits) {
+    extern __shared__ fr_t sh[];
+    ntt_tile(x, tw, k, s0, R, gbits, blockIdx.x, sh);
+}
+
+// ------------------------------------------------------------------ batched transforms
+// one (transform, pass) of a round: its tiles occupy [tile_begin, next tile_begin)
+struct NttPassDesc {
+    fr_t* x;
+    const fr_t* tw;
+    int k, s0, R, gbits;
+    uint64_t tile_begin;
+};
+// one transform for the element-wise stages: its elements occupy [begin, next begin)
+struct NttElemDesc {
+    fr_t* x;
+    const fr_t* factor;   // bit-reverse scale (1/n for inverses) or null
+    const fr_t* pre;      // coset powers applied before the passes, or null
+    const fr_t* post;     // coset powers applied after the permutation, or null
+    int k;
+    uint64_t begin;
+};
+
+template <class D>
+HZK_DEV uint32_t find_desc(const D* d, uint32_t nd, uint64_t gid) {
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].tile_begin <= gid) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+HZK_DEV uint32_t find_elem(const NttElemDesc* d, uint32_t nd, uint64_t gid) {
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d[mid].begin <= gid) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
+    extern __shared__ fr_t sh[];
+    uint32_t i = find_desc(pd, npd, blockIdx.x);
+    const NttPassDesc& d = pd[i];
+    ntt_tile(d.x, d.tw, d.k, d.s0, d.R, d.gbits, (uint32_t)(blockIdx.x - d.tile_begin), sh);
+}
+
+__global__ void __launch_bounds__(256) k_ntt_pre_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const NttElemDesc& d = ed[find_elem(ed, ne, gid)];
+    if (!d.pre) return;
+    uint64_t j = gid - d.begin;
+    d.x[j] = d.x[j] * d.pre[j];
+}
+
+__global__ void __launch_bounds__(256) k_bitrev_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const NttElemDesc& d = ed[find_elem(ed, ne, gid)];
+    size_t i = gid - d.begin;
+    size_t r = d.k ? (size_t)(__brev((uint32_t)i) >> (32 - d.k)) : 0;
+    if (i > r) return;
+    fr_t a = d.x[i], c = d.x[r];
+    if (d.factor) {
+        fr_t f = *d.factor;
+        a = a * f;
+        c = c * f;
+    }
+    if (d.post) {
+        c = c * d.post[i];
+        a = a * d.post[r];
+    }
+    d.x[i] = c;
+    d.x[r] = a;
 }
 
 // bit-reverse permutation (k bits) fused with an optional scale factor
@@ -266,6 +344,135 @@ int coset_ntt(hzk_ctx* ctx, fr_t* x, size_t deg, int inverse) {
     return 0;
 }
 
+// pass plan of ntt_run: (s0, R, gbits) per pass, the 11-stage contiguous pass last
+static void plan_passes(int k, std::vector<int3>& out) {
+    out.clear();
+    if (k == 0) return;
+    int last = k < NTT_TILE_BITS ? k : NTT_TILE_BITS;
+    int rest = k - last;
+    int npass = (rest + 8) / 9;
+    int s0 = 0;
+    for (int p = 0; p < npass; ++p) {
+        int R = rest / npass + (p < rest % npass ? 1 : 0);
+        int gbits = NTT_TILE_BITS - R;
+        int low = k - s0 - R;
+        if (gbits > low) gbits = low;
+        out.push_back(make_int3(s0, R, gbits));
+        s0 += R;
+    }
+    out.push_back(make_int3(s0, last, 0));
+}
+
+// Batched transforms: kind 0 NTT, 1 inverse NTT (scaled 1/n), 2 coset NTT,
+// 3 coset inverse NTT. Passes are right-aligned into rounds so every
+// transform's last (contiguous) pass runs in the final round; each round is one
+// launch over the tiles of every transform in it, then one launch applies the
+// bit-reverse permutation with the per-transform scale / coset factors.
+int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
+    HZK_CHECK(ctx, jobs && count > 0, HZK_ERR_INVALID, "ntt batch: empty");
+    std::vector<int> ks(count);
+    std::vector<NttTables*> tabs(count);
+    std::vector<std::vector<int3>> plans(count);
+    size_t rounds = 0;
+    for (size_t i = 0; i < count; ++i) {
+        HZK_CHECK(ctx, jobs[i].x && jobs[i].kind >= 0 && jobs[i].kind <= 3, HZK_ERR_INVALID, "ntt batch: bad job %zu", i);
+        int k = log2_exact(jobs[i].deg);
+        HZK_CHECK(ctx, k >= 0 && k <= NTT_MAX_LOG, HZK_ERR_INVALID, "ntt batch: deg %zu of job %zu is not a power of two <= 2^%d",
+                  jobs[i].deg, i, NTT_MAX_LOG);
+        ks[i] = k;
+        int rc = ensure_tables(ctx, k, &tabs[i]);
+        if (rc) return rc;
+        plan_passes(k, plans[i]);
+        rounds = std::max(rounds, plans[i].size());
+    }
+    set_ntt_lds();
+    // pass descriptors, round by round
+    std::vector<NttPassDesc> pds;
+    std::vector<size_t> round_begin(rounds + 1, 0);
+    std::vector<uint64_t> round_tiles(rounds, 0);
+    std::vector<uint32_t> round_lds(rounds, 0);
+    for (size_t r = 0; r < rounds; ++r) {
+        round_begin[r] = pds.size();
+        uint64_t tiles = 0;
+        for (size_t i = 0; i < count; ++i) {
+            size_t m = plans[i].size();
+            if (m + r < rounds) continue;
+            int3 pl = plans[i][m + r - rounds];
+            int inv = jobs[i].kind & 1;
+            NttPassDesc d{(fr_t*)jobs[i].x, tabs[i]->tw[inv], ks[i], pl.x, pl.y, pl.z, tiles};
+            pds.push_back(d);
+            tiles += (uint64_t)1 << (ks[i] - pl.y - pl.z);
+            round_lds[r] = std::max(round_lds[r], 32u << (pl.y + pl.z));
+        }
+        round_tiles[r] = tiles;
+    }
+    round_begin[rounds] = pds.size();
+    std::vector<NttElemDesc> eds(count);
+    uint64_t elems = 0;
+    bool any_pre = false;
+    for (size_t i = 0; i < count; ++i) {
+        int kind = jobs[i].kind;
+        eds[i] = {(fr_t*)jobs[i].x, (kind & 1) ? tabs[i]->ninv : nullptr, kind == 2 ? tabs[i]->coset[0] : nullptr,
+                  kind == 3 ? tabs[i]->coset[1] : nullptr, ks[i], elems};
+        any_pre |= kind == 2;
+        elems += jobs[i].deg;
+    }
+    // descriptors to the device
+    size_t pbytes = pds.size() * sizeof(NttPassDesc), ebytes = eds.size() * sizeof(NttElemDesc);
+    size_t need = ((pbytes + 255) & ~size_t(255)) + ebytes + 256;
+    if (ctx->desc_bytes < need) {
+        if (ctx->desc) { HZK_HIP(ctx, hipStreamSynchronize(ctx->stream)); HZK_HIP(ctx, hipFree(ctx->desc)); ctx->desc = nullptr; }
+        HZK_HIP(ctx, hipMalloc(&ctx->desc, need * 2));
+        ctx->desc_bytes = need * 2;
+        ++ctx->desc_generation;
+    }
+    int rc = ensure_pinned(ctx, need);
+    if (rc) return rc;
+    char* pin = (char*)ctx->pinned;
+    size_t eoff = (pbytes + 255) & ~size_t(255);
+    std::memcpy(pin, pds.data(), pbytes);
+    std::memcpy(pin + eoff, eds.data(), ebytes);
+    hipStream_t st = ctx->stream;
+    HZK_HIP(ctx, hipMemcpyAsync(ctx->desc, pin, eoff + ebytes, hipMemcpyHostToDevice, st));
+    const NttPassDesc* dpd = (const NttPassDesc*)ctx->desc;
+    const NttElemDesc* ded = (const NttElemDesc*)((char*)ctx->desc + eoff);
+
+    auto stages = [&]() {
+        if (any_pre)
+            hipLaunchKernelGGL(k_ntt_pre_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)count, elems);
+        for (size_t r = 0; r < rounds; ++r) {
+            uint32_t np = (uint32_t)(round_begin[r + 1] - round_begin[r]);
+            hipLaunchKernelGGL(k_ntt_pass_batch, dim3((unsigned)round_tiles[r]), dim3(NTT_THREADS), round_lds[r], st,
+                               dpd + round_begin[r], np);
+        }
+        hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)count, elems);
+    };
+    if (capture) {
+        std::string key = "ntt:";
+        for (size_t i = 0; i < count; ++i) key += std::to_string(ks[i]) + "/" + std::to_string(jobs[i].kind) + ";";
+        hzk_graph_entry& g = ctx->graphs[key];
+        if (!g.exec || g.generation != ctx->desc_generation) {
+            if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
+            hipGraph_t graph;
+            HZK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            stages();
+            hipError_t e = hipStreamEndCapture(st, &graph);
+            HZK_HIP(ctx, e);
+            e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HZK_HIP(ctx, e);
+            g.generation = ctx->desc_generation;
+        }
+        TimedLaunch tl(ctx, "ntt_graph");
+        HZK_HIP(ctx, hipGraphLaunch(g.exec, st));
+    } else {
+        TimedLaunch tl(ctx, "ntt_batch");
+        stages();
+    }
+    HZK_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse) { return ntt_standard(ctx, (fr_t*)x, deg, inverse); }
 int coset_ntt_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse) { return coset_ntt(ctx, (fr_t*)x, deg, inverse); }
 
@@ -292,6 +499,15 @@ int hzk_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, const void* omega_powers, in
     int k = log2_exact(deg);
     HZK_CHECK(ctx, k >= 0 && k <= NTT_MAX_LOG, HZK_ERR_INVALID, "ntt: deg %zu is not a power of two <= 2^%d", deg, NTT_MAX_LOG);
     return ntt_run(ctx, (fr_t*)coeffs, k, (const fr_t*)omega_powers, (const fr_t*)factor);
+}
+
+int hzk_ntt_batch(hzk_ctx* ctx, const hzk_ntt_desc* d, size_t count, uint32_t flags) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, d && count > 0, HZK_ERR_INVALID, "ntt batch: empty");
+    HZK_CHECK(ctx, (flags & ~HZK_BATCH_GRAPH) == 0, HZK_ERR_INVALID, "ntt batch: bad flags %u", flags);
+    std::vector<NttJob> jobs(count);
+    for (size_t i = 0; i < count; ++i) jobs[i] = {d[i].coeffs, d[i].deg, d[i].kind};
+    return ntt_batch(ctx, jobs.data(), count, (flags & HZK_BATCH_GRAPH) ? 1 : 0);
 }
 
 int hzk_coset_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, int inverse) {

@@ -114,3 +114,31 @@ def test_spmv3_and_pointwise(ctx, orc):
     # A o B - C vanishes on a satisfying witness
     z.polynomial_A_times_B_minus_C(ctx, M, outs[0], outs[1], outs[2])
     assert not outs[0].download(np.uint64, 4 * M).any()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ntt_batch_uneven_sizes_and_kinds(ctx, orc, graph):
+    """Batched transforms (one launch per pass round over all transforms,
+    optionally one hipGraph) equal the same transforms run one at a time, which
+    are pinned to the oracle above; the forward NTTs are also checked directly."""
+    rng = np.random.default_rng(9)
+    logs = [int(v) for v in rng.integers(1, 17, size=30)] + [0, 11, 12, 20]
+    kinds = [int(v) for v in rng.integers(0, 4, size=len(logs))]
+    xs = [orc.gen_fr(50 + i, 0, 1 << lg) for i, lg in enumerate(logs)]
+    for rep in range(2):
+        bufs = [ctx.upload(x) for x in xs]
+        ctx.ntt_batch([(b, 1 << lg, kd) for b, lg, kd in zip(bufs, logs, kinds)], graph=graph)
+        for i, (lg, kd) in enumerate(zip(logs, kinds)):
+            n = 1 << lg
+            ref = ctx.upload(xs[i])
+            if kd == 0:
+                ctx.ntt(ref, n)
+            elif kd == 1:
+                ctx.ntt(ref, n, inverse=True)
+            else:
+                ctx.coset_ntt(ref, n, inverse=(kd == 3))
+            got = bufs[i].download(np.uint64, 4 * n)
+            assert np.array_equal(got, ref.download(np.uint64, 4 * n)), (rep, i, lg, kd)
+            if kd == 0 and lg <= 12:
+                assert np.array_equal(got, orc.ntt(xs[i], orc.ntt_setup(n)[: 4 * n])), (i, lg)
+        xs = [orc.gen_fr(80 + i, 0, 1 << lg) for i, lg in enumerate(logs)]
