@@ -171,6 +171,39 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
 /* device pointer / count of a key array: 0 A1, 1 B1, 2 B2 (G2), 3 L1, 4 H1 */
 int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count);
 
+/* ---- proof server (no counterpart in the reference, which proves one
+ * statement per process, test/groth16.cu:3-8): several circuits' proving keys
+ * resident in HBM, requests submitted from any thread and proved by a worker
+ * with continuous batching. mode 0: each step runs the SpMV + H stage of newly
+ * admitted requests on one stream while the batched MSMs of the previous
+ * step's requests run on another; mode 1: one request at a time, no overlap.
+ * Admission is in arrival order while the requests' device footprint fits
+ * mem_budget (0 = half the free memory); a request that can never fit fails
+ * with HZK_ERR_NOMEM. A failing stage fails only its requests. A proof is a
+ * function of (circuit, z, r|s) alone: the same alone or in any mix. */
+typedef struct hzk_server hzk_server;
+int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hzk_server** out);
+void hzk_server_destroy(hzk_server* s);
+const char* hzk_server_last_error(const hzk_server* s);
+/* set up a circuit (as hzk_g16_setup) before hzk_server_start */
+int hzk_server_add_circuit(hzk_server* s, uint64_t seed, int log_m, int num_public, int* circuit_id);
+/* the circuit's hzk_g16 (witness generation, info); owned by the server */
+int hzk_server_circuit(hzk_server* s, int circuit_id, hzk_g16** out);
+/* device bytes one request of the circuit is charged against the budget */
+int hzk_server_footprint(hzk_server* s, int circuit_id, size_t* bytes);
+int hzk_server_start(hzk_server* s);
+/* z: num_vars x 32 B host (copied), rs: r | s host (copied) */
+int hzk_server_submit(hzk_server* s, int circuit_id, const void* z_host, const void* rs_host, uint64_t* request_id);
+/* 1 done (proof_out = 256 B proof, latency from submit to completion),
+ * 0 still running after timeout_s, < 0 failed; a finished request's id is
+ * released by the call that returns its result */
+int hzk_server_wait(hzk_server* s, uint64_t request_id, double timeout_s, void* proof_out, double* latency_s);
+/* {steps, completed, failed, max in flight, mean requests per MSM stage,
+ *  budget bytes, bytes admitted} */
+int hzk_server_stats(hzk_server* s, double* out7);
+/* make the n-th following MSM scratch allocation fail (-1 = off) */
+int hzk_server_inject_failure(hzk_server* s, long after_n);
+
 /* ---- timing (replaces the reference's Timer phase printouts,
  * include/timer/timer.cuh:5-12). When enabled, each kernel launch is
  * bracketed by HIP events on the context stream; totals are per kernel name. */

@@ -291,6 +291,96 @@ class Groth16:
         return p.value, n.value
 
 
+class ProofServer:
+    """Continuous-batching Groth16 proof server (hzk_server_*).
+
+    add_circuit() before start(); submit() may be called from any thread and
+    returns a request id; wait() returns (status, proof, latency_s) with status
+    1 done, 0 still running, < 0 failed (an HZK_ERR_* code)."""
+
+    def __init__(self, device=0, mem_budget=0, max_batch=64, mode="continuous"):
+        h = ctypes.c_void_p()
+        m = {"continuous": 0, "sequential": 1}[mode]
+        rc = load().hzk_server_create(device, mem_budget, max_batch, m, ctypes.byref(h))
+        if rc != 0:
+            raise HzkError(f"hzk_server_create rc={rc}")
+        self.h = h.value
+        self.circuits = []
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = load().hzk_server_last_error(self.h)
+            raise HzkError(f"rc={rc}: {msg.decode() if msg else ''}")
+
+    def add_circuit(self, seed, log_m, num_public=8):
+        cid = ctypes.c_int()
+        self._check(load().hzk_server_add_circuit(self.h, seed, log_m, num_public, ctypes.byref(cid)))
+        g = ctypes.c_void_p()
+        self._check(load().hzk_server_circuit(self.h, cid.value, ctypes.byref(g)))
+        info = (ctypes.c_uint64 * 8)()
+        load().hzk_g16_info(g.value, info)
+        keys = ["M", "num_vars", "num_public", "num_gates", "nnzA", "nnzB", "nnzC", "key_bytes"]
+        self.circuits.append((g.value, {k: int(v) for k, v in zip(keys, info)}))
+        return cid.value
+
+    def witness(self, cid, wseed):
+        g, info = self.circuits[cid]
+        z = np.empty(4 * info["num_vars"], dtype=np.uint64)
+        load().hzk_g16_witness(g, wseed, z.ctypes.data)
+        return z
+
+    def footprint(self, cid):
+        n = ctypes.c_size_t()
+        self._check(load().hzk_server_footprint(self.h, cid, ctypes.byref(n)))
+        return n.value
+
+    def start(self):
+        self._check(load().hzk_server_start(self.h))
+
+    def submit(self, cid, z, rs):
+        z = np.ascontiguousarray(z, dtype=np.uint64)
+        rs = np.ascontiguousarray(rs, dtype=np.uint64)
+        rid = ctypes.c_uint64()
+        self._check(load().hzk_server_submit(self.h, cid, z.ctypes.data, rs.ctypes.data, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=600.0):
+        proof = np.empty(32, dtype=np.uint64)
+        lat = ctypes.c_double()
+        st = load().hzk_server_wait(self.h, rid, float(timeout), proof.ctypes.data, ctypes.byref(lat))
+        return st, (proof if st == 1 else None), lat.value
+
+    def stats(self):
+        out = (ctypes.c_double * 7)()
+        self._check(load().hzk_server_stats(self.h, out))
+        keys = ["steps", "completed", "failed", "max_inflight", "mean_batch", "budget", "in_use"]
+        return {k: float(v) for k, v in zip(keys, out)}
+
+    def inject_failure(self, after_n):
+        self._check(load().hzk_server_inject_failure(self.h, after_n))
+
+    def last_error(self):
+        msg = load().hzk_server_last_error(self.h)
+        return msg.decode() if msg else ""
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().hzk_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def gen_field(seed, stream, n, kind="fr"):
     """Seeded synthetic field elements (host numpy uint64, 4 limbs each)."""
     out = np.empty(4 * n, dtype=np.uint64)

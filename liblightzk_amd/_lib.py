@@ -68,6 +68,17 @@ def load():
         "hzk_g16_rs": (i32, [ctypes.c_uint64, vp]),
         "hzk_g16_prove": (i32, [vp, vp, vp, vp, vp]),
         "hzk_g16_key_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "hzk_server_create": (i32, [i32, sz, i32, i32, ctypes.POINTER(vp)]),
+        "hzk_server_destroy": (None, [vp]),
+        "hzk_server_last_error": (ctypes.c_char_p, [vp]),
+        "hzk_server_add_circuit": (i32, [vp, ctypes.c_uint64, i32, i32, ctypes.POINTER(i32)]),
+        "hzk_server_circuit": (i32, [vp, i32, ctypes.POINTER(vp)]),
+        "hzk_server_footprint": (i32, [vp, i32, ctypes.POINTER(sz)]),
+        "hzk_server_start": (i32, [vp]),
+        "hzk_server_submit": (i32, [vp, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "hzk_server_wait": (i32, [vp, ctypes.c_uint64, dbl, vp, ctypes.POINTER(dbl)]),
+        "hzk_server_stats": (i32, [vp, ctypes.POINTER(dbl)]),
+        "hzk_server_inject_failure": (i32, [vp, i64]),
         "hzk_timing_enable": (i32, [vp, i32]),
         "hzk_timing_reset": (i32, [vp]),
         "hzk_timing_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]),

@@ -52,16 +52,32 @@ int ensure_scratch(hzk_ctx* ctx, size_t bytes) {
     return 0;
 }
 
-int ensure_pinned(hzk_ctx* ctx, size_t bytes) {
-    if (ctx->pinned_bytes >= bytes) return 0;
-    if (ctx->pinned) {
-        HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        HZK_HIP(ctx, hipHostFree(ctx->pinned));
-        ctx->pinned = nullptr;
+int stage_acquire(hzk_ctx* ctx, size_t bytes, char** buf, int* slot) {
+    int i = ctx->stage_next;
+    ctx->stage_next = (i + 1) % hzk_ctx::STAGE_SLOTS;
+    hzk_ctx::Stage& st = ctx->stage[i];
+    if (st.fenced) {
+        HZK_HIP(ctx, hipEventSynchronize(st.done));
+        st.fenced = false;
     }
-    size_t want = bytes < 65536 ? 65536 : bytes;
-    HZK_HIP(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
-    ctx->pinned_bytes = want;
+    if (!st.done) HZK_HIP(ctx, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    if (st.bytes < bytes) {
+        if (st.p) HZK_HIP(ctx, hipHostFree(st.p));
+        st.p = nullptr;
+        st.bytes = 0;
+        size_t want = bytes < 65536 ? 65536 : bytes + bytes / 4;
+        HZK_HIP(ctx, hipHostMalloc(&st.p, want, hipHostMallocDefault));
+        st.bytes = want;
+    }
+    *buf = (char*)st.p;
+    *slot = i;
+    return 0;
+}
+
+int stage_fence(hzk_ctx* ctx, int slot) {
+    hzk_ctx::Stage& st = ctx->stage[slot];
+    HZK_HIP(ctx, hipEventRecord(st.done, ctx->stream));
+    st.fenced = true;
     return 0;
 }
 
@@ -171,7 +187,10 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     if (ctx->desc) (void)hipFree(ctx->desc);
     if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);
     if (ctx->comb_g2) (void)hipFree(ctx->comb_g2);
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto& st : ctx->stage) {
+        if (st.p) (void)hipHostFree(st.p);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -37,9 +37,13 @@ struct hzk_ctx {
     uint64_t desc_generation = 0;
     // captured device-stage graphs keyed by batch shape
     std::map<std::string, hzk_graph_entry> graphs;
-    // pinned host staging for small results
-    void* pinned = nullptr;
-    size_t pinned_bytes = 0;
+    // pinned host staging slots, used round-robin: a slot is handed out again
+    // only after the copies that read it (fenced by its event) have run, so
+    // back-to-back batched calls never overwrite descriptors still in flight
+    struct Stage { void* p = nullptr; size_t bytes = 0; hipEvent_t done = nullptr; bool fenced = false; };
+    static constexpr int STAGE_SLOTS = 4;
+    Stage stage[STAGE_SLOTS];
+    int stage_next = 0;
     // timing: when enabled every kernel launch is bracketed by events
     bool timing = false;
     struct Rec { hipEvent_t a, b; std::string name; };
@@ -90,7 +94,10 @@ struct Arena {
 };
 // Make sure ctx->scratch holds at least `bytes`; returns 0 or an error code.
 int ensure_scratch(hzk_ctx* ctx, size_t bytes);
-int ensure_pinned(hzk_ctx* ctx, size_t bytes);
+// Pinned host staging of at least `bytes`: *buf is writable now; call
+// stage_fence(ctx, slot) after enqueueing the copies that read it.
+int stage_acquire(hzk_ctx* ctx, size_t bytes, char** buf, int* slot);
+int stage_fence(hzk_ctx* ctx, int slot);
 void ntt_release(hzk_ctx* ctx);
 int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, const void* z, void* Az, void* Bz, void* Cz);
 int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale);

@@ -1,0 +1,56 @@
+// Groth16 prover internals shared by the plain prover (groth16.cpp) and the
+// serving loop (server.cpp).
+#pragma once
+#include <vector>
+#include "common.hpp"
+#include "host_ec.hpp"
+#include "msm.hpp"
+
+namespace hzk {
+struct HostCsr {
+    std::vector<uint64_t> row_ptr, col;
+    std::vector<host::Fr> val;
+};
+
+}  // namespace hzk
+
+struct hzk_g16 {
+    int log_m = 0;
+    size_t M = 0, num_vars = 0, num_public = 0, num_gates = 0;
+    hzk::HostCsr m[3];
+    // device copies
+    void* d_val[3] = {nullptr, nullptr, nullptr};
+    uint64_t* d_col[3] = {nullptr, nullptr, nullptr};
+    uint64_t* d_rp[3] = {nullptr, nullptr, nullptr};
+    // proving key in HBM (affine)
+    void *A1 = nullptr, *B1 = nullptr, *B2 = nullptr, *L1 = nullptr, *H1 = nullptr;
+    hzk::host::G1x alpha1, beta1, delta1;
+    hzk::host::G2x beta2, gamma2, delta2;
+    std::vector<hzk::host::G1x> IC;   // verifying-key input commitments
+    // buffers of the plain one-proof-at-a-time prover, and 1/(g^M - 1)
+    void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *zinv = nullptr;
+    std::vector<void*> owned;
+    int device = 0;
+};
+
+namespace hzk {
+
+// One proof in flight: device buffers for z and the three evaluation vectors
+// (a holds h after the H stage), the five MSM results, randomness and output.
+struct G16Req {
+    hzk_g16* g;
+    void *z, *a, *b, *c;
+    char res[576];   // A1, B1 (G1, 96 B each), B2 (G2, 192 B), L, H (G1)
+};
+// SpMV and the H pipeline (3 iNTT, 3 coset NTT, (AB - C)/Z, coset iNTT) for
+// several requests, enqueued on ctx's stream with batched transforms.
+int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n);
+// the five MSMs of several requests as two batched MSMs (G1: 4 per request,
+// G2: 1 per request); results land in reqs[i].res on the host
+int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph);
+// phase-6 assembly on the host; proof = A | B | C (affine, 256 B)
+void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof);
+// device bytes one request needs beyond its buffers (MSM scratch, generous)
+size_t g16_request_footprint(const hzk_g16* g);
+
+}  // namespace hzk

@@ -29,6 +29,7 @@
 #include "host_ec.hpp"
 #include "msm.hpp"
 #include "synth.hpp"
+#include "g16.hpp"
 
 using namespace hzk;
 using host::Fr;
@@ -67,31 +68,7 @@ Fr fr_pow_u64(Fr b, uint64_t e) {
     return acc;
 }
 
-struct HostCsr {
-    std::vector<uint64_t> row_ptr, col;
-    std::vector<Fr> val;
-};
-
 }  // namespace
-
-struct hzk_g16 {
-    int log_m = 0;
-    size_t M = 0, num_vars = 0, num_public = 0, num_gates = 0;
-    HostCsr m[3];
-    // device copies
-    void* d_val[3] = {nullptr, nullptr, nullptr};
-    uint64_t* d_col[3] = {nullptr, nullptr, nullptr};
-    uint64_t* d_rp[3] = {nullptr, nullptr, nullptr};
-    // proving key in HBM (affine)
-    void *A1 = nullptr, *B1 = nullptr, *B2 = nullptr, *L1 = nullptr, *H1 = nullptr;
-    G1x alpha1, beta1, delta1;
-    G2x beta2, gamma2, delta2;
-    std::vector<G1x> IC;   // verifying-key input commitments
-    // per-proof device buffers
-    void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *res = nullptr, *zinv = nullptr;
-    std::vector<void*> owned;
-    int device = 0;
-};
 
 namespace {
 
@@ -227,7 +204,6 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->a);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->b);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->c);
-    if (!rc) rc = dev_alloc(ctx, g, 5 * 192, &g->res);
     if (rc) return rc;
     Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
     return dev_upload(ctx, g, zi.v, 32, &g->zinv);
@@ -237,31 +213,67 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
 
 namespace hzk {
 
-// Device phases of one proof over z already in g->z: SpMV, H, five MSMs.
-// Writes five normalised Jacobian points into host res[5 * 192].
-int g16_device_phases(hzk_ctx* ctx, hzk_g16* g, char* res) {
-    const size_t M = g->M, N1 = g->num_vars, l = g->num_public;
-    hzk_csr cs[3];
-    for (int i = 0; i < 3; ++i)
-        cs[i] = {g->d_val[i], g->d_col[i], g->d_rp[i], M, N1, g->m[i].col.size()};
-    int rc = spmv3(ctx, &cs[0], &cs[1], &cs[2], g->z, g->a, g->b, g->c);
-    for (void* v : {g->a, g->b, g->c}) {
-        if (!rc) rc = ntt_standard_raw(ctx, v, M, 1);
-        if (!rc) rc = coset_ntt_raw(ctx, v, M, 0);
+int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n) {
+    std::vector<NttJob> jobs;
+    for (size_t i = 0; i < n; ++i) {
+        hzk_g16* g = reqs[i].g;
+        hzk_csr cs[3];
+        for (int m = 0; m < 3; ++m) cs[m] = {g->d_val[m], g->d_col[m], g->d_rp[m], g->M, g->num_vars, g->m[m].col.size()};
+        int rc = spmv3(ctx, &cs[0], &cs[1], &cs[2], reqs[i].z, reqs[i].a, reqs[i].b, reqs[i].c);
+        if (rc) return rc;
     }
-    if (!rc) rc = pointwise_abc(ctx, M, g->a, g->b, g->c, g->zinv);
-    if (!rc) rc = coset_ntt_raw(ctx, g->a, M, 1);
-    if (rc) return rc;
-    char* d = (char*)g->res;
-    rc = msm_g1(ctx, g->z, g->A1, N1, 0, d);
-    if (!rc) rc = msm_g1(ctx, g->z, g->B1, N1, 0, d + 96);
-    if (!rc) rc = msm_g2(ctx, g->z, g->B2, N1, 0, d + 192);
-    if (!rc) rc = msm_g1(ctx, (char*)g->z + (l + 1) * 32, g->L1, N1 - l - 1, 0, d + 384);
-    if (!rc) rc = msm_g1(ctx, g->a, g->H1, M - 1, 0, d + 480);
-    if (rc) return rc;
-    HZK_HIP(ctx, hipMemcpyAsync(res, g->res, 576, hipMemcpyDeviceToHost, ctx->stream));
-    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return 0;
+    for (int kind : {1, 2}) {
+        jobs.clear();
+        for (size_t i = 0; i < n; ++i)
+            for (void* v : {reqs[i].a, reqs[i].b, reqs[i].c}) jobs.push_back({v, reqs[i].g->M, kind});
+        int rc = ntt_batch(ctx, jobs.data(), jobs.size(), 0);
+        if (rc) return rc;
+    }
+    jobs.clear();
+    for (size_t i = 0; i < n; ++i) {
+        int rc = pointwise_abc(ctx, reqs[i].g->M, reqs[i].a, reqs[i].b, reqs[i].c, reqs[i].g->zinv);
+        if (rc) return rc;
+        jobs.push_back({reqs[i].a, reqs[i].g->M, 3});
+    }
+    return ntt_batch(ctx, jobs.data(), jobs.size(), 0);
+}
+
+int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
+    std::vector<MsmJob> j1, j2;
+    for (size_t i = 0; i < n; ++i) {
+        hzk_g16* g = reqs[i].g;
+        const size_t N1 = g->num_vars, l = g->num_public;
+        char* r = reqs[i].res;
+        j1.push_back({reqs[i].z, g->A1, N1, r, 0});
+        j1.push_back({reqs[i].z, g->B1, N1, r + 96, 0});
+        j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1, N1 - l - 1, r + 384, 0});
+        j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0});
+        j2.push_back({reqs[i].z, g->B2, N1, r + 192, 0});
+    }
+    int rc = msm_batch_g1(ctx, j1.data(), j1.size(), 1, graph);
+    if (!rc) rc = msm_batch_g2(ctx, j2.data(), j2.size(), 1, graph);
+    return rc;
+}
+
+// Upper bound of one MSM's scratch (sorted digits, histograms, key scans,
+// accumulation segments, bucket-reduction and window sums) for n points with
+// xyzz points of pt_bytes, from the built-in plan.
+static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
+    MsmPlan p = msm_plan(n, 0, 0, 160 * 1024);
+    size_t W = p.W, nk = W * p.NB, entries = n * W;
+    size_t segs = (entries + p.K - 1) / p.K + nk;
+    size_t bred = W * ((p.NB + p.g - 1) / p.g);
+    return entries * 8 + (size_t)p.T * nk * 4 + 3 * (nk + 1) * 4 + 2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 +
+           segs * (4 + pt_bytes) + (bred + W) * pt_bytes + 4096;
+}
+
+size_t g16_request_footprint(const hzk_g16* g) {
+    // z + three evaluation vectors, and the scratch of its five MSMs with 25% slack
+    size_t bufs = (g->num_vars + 3 * g->M) * 32;
+    size_t N1 = g->num_vars, L = N1 - g->num_public - 1;
+    size_t msm = 2 * msm_scratch_bound(N1, 128) + msm_scratch_bound(L, 128) + msm_scratch_bound(g->M - 1, 128) +
+                 msm_scratch_bound(N1, 256);
+    return bufs + msm + msm / 4;
 }
 
 // Phase 6 on the host; proof = A (64 B) | B (128 B) | C (64 B), affine.
@@ -359,10 +371,11 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
     if (!ctx || !g) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, z_host && rs_host && proof_host, HZK_ERR_INVALID, "g16_prove: null argument");
     HZK_HIP(ctx, hipMemcpyAsync(g->z, z_host, g->num_vars * 32, hipMemcpyHostToDevice, ctx->stream));
-    char res[576];
-    int rc = g16_device_phases(ctx, g, res);
+    G16Req r{g, g->z, g->a, g->b, g->c, {}};
+    int rc = g16_h_batch(ctx, &r, 1);
+    if (!rc) rc = g16_msm_batch(ctx, &r, 1, 0);
     if (rc) return rc;
-    g16_assemble(g, res, rs_host, proof_host);
+    g16_assemble(g, r.res, rs_host, proof_host);
     return HZK_OK;
 }
 

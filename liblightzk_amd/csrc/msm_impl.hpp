@@ -334,12 +334,13 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     int rc = ensure_scratch(ctx, L.bytes);
     if (rc) return rc;
     const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 + 1024;
-    rc = ensure_pinned(ctx, host_bytes);
+    char* pin;
+    int slot;
+    rc = stage_acquire(ctx, host_bytes, &pin, &slot);
     if (rc) return rc;
     char* base = (char*)ctx->scratch;
     hipStream_t st = ctx->stream;
     // descriptors and work-space starts
-    char* pin = (char*)ctx->pinned;
     std::memcpy(pin, L.inst.data(), count * sizeof(MsmInst));
     std::memcpy(pin + count * sizeof(MsmInst), L.begin.data(), L.begin.size() * 8);
     HZK_HIP(ctx, hipMemcpyAsync(base + L.o_inst, pin, count * sizeof(MsmInst), hipMemcpyHostToDevice, st));
@@ -373,6 +374,8 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     uint32_t* hstats = (uint32_t*)(hwin + L.tot[SP_WIN] * sizeof(xyzz_t<F>));
     HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
     HZK_HIP(ctx, hipMemcpyAsync(hstats, base + L.o_stats, count * 16, hipMemcpyDeviceToHost, st));
+    rc = stage_fence(ctx, slot);
+    if (rc) return rc;
     HZK_HIP(ctx, hipStreamSynchronize(st));
 
     // window combine (Horner over windows) on the host, instances in parallel

@@ -426,14 +426,17 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         ctx->desc_bytes = need * 2;
         ++ctx->desc_generation;
     }
-    int rc = ensure_pinned(ctx, need);
+    char* pin;
+    int slot;
+    int rc = stage_acquire(ctx, need, &pin, &slot);
     if (rc) return rc;
-    char* pin = (char*)ctx->pinned;
     size_t eoff = (pbytes + 255) & ~size_t(255);
     std::memcpy(pin, pds.data(), pbytes);
     std::memcpy(pin + eoff, eds.data(), ebytes);
     hipStream_t st = ctx->stream;
     HZK_HIP(ctx, hipMemcpyAsync(ctx->desc, pin, eoff + ebytes, hipMemcpyHostToDevice, st));
+    rc = stage_fence(ctx, slot);
+    if (rc) return rc;
     const NttPassDesc* dpd = (const NttPassDesc*)ctx->desc;
     const NttElemDesc* ded = (const NttElemDesc*)((char*)ctx->desc + eoff);
 
