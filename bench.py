@@ -90,6 +90,96 @@ def groth16_plain(ctx, log_m, proofs):
             "kernels_ms_per_proof": {k: v[0] / proofs for k, v in rep.items()}}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(len(xs) * q))]
+
+
+def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5):
+    """Mixed-size request stream served two ways by one hzk_server holding all
+    circuits' keys: continuous batching (mode 0) and the plain sequential
+    prover through the same queue (mode 1). Requests arrive as a seeded Poisson
+    stream at rate_mult x the plain prover's measured capacity on this mix;
+    each carries its own seeded witness (generated before the clock starts).
+    Both runs must return the same proof bytes request by request."""
+    import threading
+    import liblightzk_amd as z
+    srv = z.ProofServer(0, max_batch=max_batch, mode="sequential")
+    t = time.perf_counter()
+    cids = [srv.add_circuit(40 + lg, lg, 8) for lg in logs]
+    setup_s = time.perf_counter() - t
+    log(f"served: setup of {len(logs)} circuits {setup_s:.1f} s")
+    rng = np.random.default_rng(seed)
+    p = np.array(weights, dtype=float) / sum(weights)
+    kinds = rng.choice(len(logs), size=requests, p=p)
+    gaps = rng.exponential(1.0, size=requests)
+    reqs = [(int(k), srv.witness(cids[k], 1000 + i), z.Groth16.rs(1000 + i)) for i, k in enumerate(kinds)]
+    srv.start()
+    # plain per-circuit proof time (one warm request each, then 2 timed)
+    plain_t = []
+    for k in range(len(logs)):
+        zz, rs = srv.witness(cids[k], 7), z.Groth16.rs(7)
+        srv.wait(srv.submit(cids[k], zz, rs))
+        ts = []
+        for _ in range(2):
+            a = time.perf_counter()
+            srv.wait(srv.submit(cids[k], zz, rs))
+            ts.append(time.perf_counter() - a)
+        plain_t.append(min(ts))
+    capacity = 1.0 / float(np.dot(p, plain_t))
+    rate = rate_mult * capacity
+    arrivals = np.cumsum(gaps / rate)
+
+    def run(mode):
+        srv.set_mode(mode)
+        ids = [None] * requests
+        t0 = time.perf_counter()
+
+        def client():
+            for i, (k, zz, rs) in enumerate(reqs):
+                d = t0 + arrivals[i] - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[i] = srv.submit(cids[k], zz, rs)
+
+        th = threading.Thread(target=client)
+        th.start()
+        out, lat = [], []
+        for i in range(requests):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            st, proof, l = srv.wait(ids[i], 600)
+            if st != 1:
+                raise RuntimeError(f"served request {i} failed: {st} {srv.last_error()}")
+            out.append(proof)
+            lat.append(l)
+        el = time.perf_counter() - t0
+        th.join()
+        return out, lat, el
+
+    res = {}
+    proofs = {}
+    for mode in ("sequential", "continuous"):
+        st0 = srv.stats()
+        proofs[mode], lat, el = run(mode)
+        st1 = srv.stats()
+        res[mode] = {"proofs_per_s": requests / el, "p50_ms": _pct(lat, 0.5) * 1e3, "p99_ms": _pct(lat, 0.99) * 1e3,
+                     "wall_s": el, "steps": st1["steps"] - st0["steps"], "max_inflight": st1["max_inflight"]}
+        log(f"served: {mode} {res[mode]['proofs_per_s']:.1f} proofs/s p99 {res[mode]['p99_ms']:.0f} ms")
+    same = all(np.array_equal(a, b) for a, b in zip(proofs["sequential"], proofs["continuous"]))
+    srv.close()
+    return {"circuits": [f"m=2^{lg}" for lg in logs], "weights": list(weights), "requests": requests,
+            "arrivals": f"Poisson, seed {seed}, rate {rate:.1f}/s = {rate_mult}x plain capacity {capacity:.1f}/s",
+            "plain_ms_per_circuit": [x * 1e3 for x in plain_t], "setup_s": setup_s, "max_batch": max_batch,
+            "sequential": res["sequential"], "continuous": res["continuous"],
+            "speedup_proofs_per_s": res["continuous"]["proofs_per_s"] / res["sequential"]["proofs_per_s"],
+            "proofs_identical_across_modes": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +190,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--g16-log-m", type=int, default=18)
     ap.add_argument("--g16-proofs", type=int, default=5)
+    ap.add_argument("--serve-logs", default="12,16,18,20")
+    ap.add_argument("--serve-weights", default="40,30,20,10")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate-mult", type=float, default=4.0)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU only "
@@ -173,8 +268,14 @@ def main():
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in rep.items()},
         "roofline": roofline,
     }
+    log(f"msm: {value:.1f} Mpoints/s")
     if args.g16_proofs > 0:
         out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs)
+        log(f"groth16 plain: {out['groth16_plain']['proofs_per_s']:.1f} proofs/s")
+    if args.serve_requests > 0:
+        out["groth16_served"] = groth16_served([int(v) for v in args.serve_logs.split(",")],
+                                               [float(v) for v in args.serve_weights.split(",")],
+                                               args.serve_requests, args.serve_rate_mult, args.serve_max_batch)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scalars, d_b, args.cpu_seconds)
     print(json.dumps(out), flush=True)

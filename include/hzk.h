@@ -192,6 +192,8 @@ int hzk_server_circuit(hzk_server* s, int circuit_id, hzk_g16** out);
 /* device bytes one request of the circuit is charged against the budget */
 int hzk_server_footprint(hzk_server* s, int circuit_id, size_t* bytes);
 int hzk_server_start(hzk_server* s);
+/* switch between mode 0 and 1 while no request is pending or in flight */
+int hzk_server_set_mode(hzk_server* s, int mode);
 /* z: num_vars x 32 B host (copied), rs: r | s host (copied) */
 int hzk_server_submit(hzk_server* s, int circuit_id, const void* z_host, const void* rs_host, uint64_t* request_id);
 /* 1 done (proof_out = 256 B proof, latency from submit to completion),

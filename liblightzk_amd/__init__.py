@@ -337,6 +337,9 @@ class ProofServer:
     def start(self):
         self._check(load().hzk_server_start(self.h))
 
+    def set_mode(self, mode):
+        self._check(load().hzk_server_set_mode(self.h, {"continuous": 0, "sequential": 1}[mode]))
+
     def submit(self, cid, z, rs):
         z = np.ascontiguousarray(z, dtype=np.uint64)
         rs = np.ascontiguousarray(rs, dtype=np.uint64)

@@ -75,6 +75,7 @@ def load():
         "hzk_server_circuit": (i32, [vp, i32, ctypes.POINTER(vp)]),
         "hzk_server_footprint": (i32, [vp, i32, ctypes.POINTER(sz)]),
         "hzk_server_start": (i32, [vp]),
+        "hzk_server_set_mode": (i32, [vp, i32]),
         "hzk_server_submit": (i32, [vp, i32, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
         "hzk_server_wait": (i32, [vp, ctypes.c_uint64, dbl, vp, ctypes.POINTER(dbl)]),
         "hzk_server_stats": (i32, [vp, ctypes.POINTER(dbl)]),

@@ -193,15 +193,17 @@ int stage_msm(hzk_server* s, std::vector<Request*>& rs) {
 void worker_loop(hzk_server* s) {
     (void)hipSetDevice(s->device);
     std::vector<Request*> stage1, stage2;
+    int mode = 0;
     for (;;) {
         std::vector<Request*> admitted;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !stage1.empty(); });
             if (s->stop && s->pending.empty() && stage1.empty()) break;
-            size_t cap = s->mode == 1 ? 1 : (size_t)s->max_batch;
+            mode = s->mode;
+            size_t cap = mode == 1 ? 1 : (size_t)s->max_batch;
             size_t inflight = stage1.size();
-            while (!s->pending.empty() && admitted.size() + (s->mode == 1 ? inflight : 0) < cap) {
+            while (!s->pending.empty() && admitted.size() + (mode == 1 ? inflight : 0) < cap) {
                 Request* r = s->pending.front();
                 if (r->footprint > s->budget) {   // can never fit: fail it
                     s->pending.pop_front();
@@ -233,7 +235,7 @@ void worker_loop(hzk_server* s) {
             std::lock_guard<std::mutex> lk(s->mu);
             ++s->steps;
         }
-        if (s->mode == 1) {
+        if (mode == 1) {
             // plain: one request through both stages, nothing overlapped
             const hzk_ctx* ec = s->ctx_h;
             int rc = stage_h(s, ok);
@@ -313,6 +315,17 @@ int hzk_server_circuit(hzk_server* s, int circuit_id, hzk_g16** out) {
 int hzk_server_footprint(hzk_server* s, int circuit_id, size_t* bytes) {
     if (!s || !bytes || circuit_id < 0 || circuit_id >= (int)s->circuits.size()) return HZK_ERR_INVALID;
     *bytes = g16_request_footprint(s->circuits[circuit_id]);
+    return HZK_OK;
+}
+
+int hzk_server_set_mode(hzk_server* s, int mode) {
+    if (!s || (mode != 0 && mode != 1)) return HZK_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->pending.empty() || s->in_use != 0) {
+        s->last_error = "set_mode: requests in flight";
+        return HZK_ERR_INVALID;
+    }
+    s->mode = mode;
     return HZK_OK;
 }
 
