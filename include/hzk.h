@@ -45,6 +45,9 @@ int hzk_ctx_create(int device, void* hip_stream /* nullable: own stream */, hzk_
 void hzk_ctx_destroy(hzk_ctx* ctx);
 const char* hzk_last_error(const hzk_ctx* ctx);
 int hzk_sync(hzk_ctx* ctx);
+/* wait for all work on the context's device, every stream (the reference's
+ * operators run on the legacy default stream; its callers rely on that order) */
+int hzk_device_sync(hzk_ctx* ctx);
 void* hzk_ctx_stream(hzk_ctx* ctx);
 const char* hzk_version(void);
 
@@ -71,6 +74,11 @@ int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, 
  * written normalised; dest may alias other buffers of the call.
  * win_siz: window bits c (0 = autotuned). ---- */
 #define HZK_BASES_AFFINE 0u
+/* the reference's pre_comp table (include/operator/msm.cuh:48-75): G1 96-B
+ * Jacobian records X|Y|Z with Z = 1; G2 two 96-B lane records per point
+ * [X.c0|Y.c0|Z.c0][X.c1|Y.c1|Z.c1] (alt_bn128_t.cuh:244-256); level-major.
+ * The MSM reads level 0 (the first n points) only. */
+#define HZK_BASES_REF_LEVELS 1u
 int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
                size_t win_siz, void* dest);
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
@@ -92,7 +100,7 @@ int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t
 int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
 
 /* statistics of the last MSM on this context: {bucket entries (nonzero
- * digits), accumulation segments, largest bucket, window bits c, windows} */
+ * digits), accumulation chunks, largest bucket, window bits c, windows} */
 int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5);
 
 /* ---- NTT over Fr (replaces number_theory_transformation,
@@ -144,6 +152,15 @@ int hzk_ntt_batch(hzk_ctx* ctx, const hzk_ntt_desc* d, size_t count, uint32_t fl
  * scalars are not modified (the reference converts them in place, msm.cuh:16). */
 int hzk_fixbase_g1(hzk_ctx* ctx, const void* scalars, size_t n, void* out_affine);
 int hzk_fixbase_g2(hzk_ctx* ctx, const void* scalars, size_t n, void* out_affine);
+/* convert n base points between layouts (HZK_BASES_AFFINE <-> HZK_BASES_REF_LEVELS,
+ * level 0); group 1 = G1, 2 = G2 */
+int hzk_bases_convert(hzk_ctx* ctx, int group, uint32_t from_layout, uint32_t to_layout, const void* src, void* dst,
+                      size_t n);
+/* replaces pre_comp_g1/_g2 (include/operator/msm.cuh:48-75): normalises the
+ * `length` Jacobian records of level 0 in place (Z = 1; infinity -> (0, 0, 1)
+ * as the reference leaves it) and writes levels j = 1..level-1 as
+ * 2^(gap*j) * P_i at record j*length + i (G2: point j*length + i) */
+int hzk_precomp(hzk_ctx* ctx, int group, void* points, size_t length, size_t gap, size_t level);
 
 /* ---- seeded synthetic inputs (replaces the reference's std::random_device draws,
  * include/host-arith/field/bigint.cuh:172-182, and Fp_model::random_element,

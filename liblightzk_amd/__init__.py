@@ -208,7 +208,7 @@ class Context:
     def msm_last_stats(self):
         a = (ctypes.c_uint32 * 5)()
         self._check(load().hzk_msm_last_stats(self.handle, a))
-        return dict(zip(("entries", "segments", "max_bucket", "c", "windows"), list(a)))
+        return dict(zip(("entries", "chunks", "max_bucket", "c", "windows"), list(a)))
 
     def fixbase_g1(self, scalars, n, out_affine):
         self._check(load().hzk_fixbase_g1(self.handle, int(scalars), n, int(out_affine)))

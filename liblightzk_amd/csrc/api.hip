@@ -187,6 +187,7 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     if (ctx->desc) (void)hipFree(ctx->desc);
     if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);
     if (ctx->comb_g2) (void)hipFree(ctx->comb_g2);
+    if (ctx->conv) (void)hipFree(ctx->conv);
     for (auto& st : ctx->stage) {
         if (st.p) (void)hipHostFree(st.p);
         if (st.done) (void)hipEventDestroy(st.done);
@@ -201,6 +202,13 @@ void* hzk_ctx_stream(hzk_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; 
 int hzk_sync(hzk_ctx* ctx) {
     if (!ctx) return HZK_ERR_INVALID;
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HZK_OK;
+}
+
+int hzk_device_sync(hzk_ctx* ctx) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_HIP(ctx, hipSetDevice(ctx->device));
+    HZK_HIP(ctx, hipDeviceSynchronize());
     return HZK_OK;
 }
 
@@ -269,13 +277,23 @@ int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, 
 int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
                void* dest) {
     if (!ctx) return HZK_ERR_INVALID;
-    HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g1: base layout %u unsupported", base_layout);
+    HZK_CHECK(ctx, base_layout <= HZK_BASES_REF_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g1: base layout %u unsupported", base_layout);
+    if (base_layout == HZK_BASES_REF_LEVELS) {
+        HZK_CHECK(ctx, bases && n > 0, HZK_ERR_INVALID, "msm_g1: null bases or n = 0");
+        int rc = ref_bases_affine(ctx, 1, bases, n, &bases);
+        if (rc) return rc;
+    }
     return msm_g1(ctx, scalars, bases, n, win_siz, dest);
 }
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
                void* dest) {
     if (!ctx) return HZK_ERR_INVALID;
-    HZK_CHECK(ctx, base_layout == HZK_BASES_AFFINE, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
+    HZK_CHECK(ctx, base_layout <= HZK_BASES_REF_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
+    if (base_layout == HZK_BASES_REF_LEVELS) {
+        HZK_CHECK(ctx, bases && n > 0, HZK_ERR_INVALID, "msm_g2: null bases or n = 0");
+        int rc = ref_bases_affine(ctx, 2, bases, n, &bases);
+        if (rc) return rc;
+    }
     return msm_g2(ctx, scalars, bases, n, win_siz, dest);
 }
 

@@ -55,6 +55,9 @@ struct hzk_ctx {
     // fixed-base comb tables (device, built on first use)
     void* comb_g1 = nullptr;
     void* comb_g2 = nullptr;
+    // affine copy of a reference-layout base table (layout.hip)
+    void* conv = nullptr;
+    size_t conv_bytes = 0;
     // last MSM: entries, segments, largest bucket, window bits, windows
     uint32_t last_msm[5] = {0, 0, 0, 0, 0};
     // NTT twiddle / coset tables for the last transform size (ntt.hip)
@@ -109,6 +112,8 @@ struct NttJob {
     int kind;   // 0 NTT, 1 inverse NTT, 2 coset NTT, 3 coset inverse NTT
 };
 int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture);
+// reference base tables (layout.hip): level 0 as affine in ctx->conv
+int ref_bases_affine(hzk_ctx* ctx, int group, const void* ref, size_t n, const void** out);
 int coset_ntt_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 
 // Kernel timing helpers (no-ops unless ctx->timing).

@@ -15,14 +15,15 @@
 //              (2^(c-1) counters, <= 128 KiB), one coalesced row per tile. No
 //              device-scope atomics: on a multi-XCD part they resolve past the
 //              per-XCD L2 and ran at ~5 G/s, slower than the whole accumulation.
-//  3. scan     per key a prefix over tiles, then an exclusive scan of
-//              (count, ceil(count/K)) -> bucket offsets and level-1 segment
-//              bases; per-instance totals land in device stats, so no stage
-//              waits for the host.
+//  3. scan     per key a prefix over tiles, then an exclusive scan of the
+//              bucket counts -> bucket offsets; per-instance totals (entries,
+//              chunks, largest bucket) land in device stats, so no stage waits
+//              for the host.
 //  4. scatter  same (tile, window) workgroups: LDS cursors seeded from the
 //              scan, LDS atomics hand out slots, point index | sign<<31 is
 //              written into its bucket.
-//  5-8 (msm_impl.hpp) accumulate, merge, bucket reduction, window sums;
+//  5-8 (msm_impl.hpp) chunked accumulation with boundary fix-up, bucket
+//              reduction, window sums;
 //  9. window combine on the host (Horner with c doublings per window) and
 //              normalisation; see host_ec.hpp for why this tail runs on the CPU.
 //
@@ -127,9 +128,6 @@ __global__ void __launch_bounds__(256) k_msm_colscan(MsmBatchDev b, uint32_t* __
     count[gid] = run;
 }
 
-// packed value: low 32 bits = bucket count, high 32 bits = ceil(count / K)
-HZK_DEV uint64_t pack_count(uint32_t c, uint32_t K) { return ((uint64_t)((c + K - 1) / K) << 32) | c; }
-
 // one block per scan tile: tile sum and tile maximum
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(MsmBatchDev b, const uint32_t* __restrict__ count,
                                                              uint64_t* __restrict__ tile_sum, uint64_t* __restrict__ tile_max) {
@@ -144,7 +142,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(MsmBatchDev b, cons
     uint32_t mx = 0;
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + threadIdx.x + t * SCAN_THREADS;
-        if (k <= d.nk) { uint32_t c = count[kbase + k]; s += pack_count(c, d.K); mx = max(mx, c); }
+        if (k <= d.nk) { uint32_t c = count[kbase + k]; s += c; mx = max(mx, c); }
     }
     red[threadIdx.x] = s;
     redmax[threadIdx.x] = mx;
@@ -189,17 +187,17 @@ __global__ void __launch_bounds__(1024) k_scan_top(MsmBatchDev b, uint64_t* __re
     uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
     for (uint32_t i = lo; i < hi; ++i) { uint64_t v = tile_sum[t0 + i]; tile_sum[t0 + i] = run; run += v; }
     if (threadIdx.x == 1023) {
-        b.stats[4 * ii + 0] = (uint32_t)part[1023];
-        b.stats[4 * ii + 1] = (uint32_t)(part[1023] >> 32);
+        const uint32_t entries = (uint32_t)part[1023], K = b.inst[ii].K;
+        b.stats[4 * ii + 0] = entries;
+        b.stats[4 * ii + 1] = (entries + K - 1) / K;
         b.stats[4 * ii + 2] = pmax[1023];
         b.stats[4 * ii + 3] = 0;
     }
 }
 
-// per tile: exclusive scan; bucket offsets and segment bases for keys 0..nk
+// per tile: exclusive scan -> bucket offsets for keys 0..nk
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(MsmBatchDev b, const uint32_t* __restrict__ count,
-                                                             const uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ offs,
-                                                             uint32_t* __restrict__ segbase) {
+                                                             const uint64_t* __restrict__ tile_sum, uint32_t* __restrict__ offs) {
     __shared__ uint64_t part[SCAN_THREADS];
     const uint64_t* tbeg = space(b, SP_SCANTILE);
     uint32_t ii = find_inst(tbeg, b.ninst, blockIdx.x);
@@ -210,7 +208,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(MsmBatchDev b, cons
     uint64_t s = 0;
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + t;
-        v[t] = k <= d.nk ? pack_count(count[kbase + k], d.K) : 0;
+        v[t] = k <= d.nk ? count[kbase + k] : 0;
         s += v[t];
     }
     part[threadIdx.x] = s;
@@ -224,10 +222,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(MsmBatchDev b, cons
     uint64_t run = tile_sum[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
     for (int t = 0; t < SCAN_ITEMS; ++t) {
         uint32_t k = base + t;
-        if (k <= d.nk) {
-            offs[kbase + k] = (uint32_t)run;
-            segbase[kbase + k] = (uint32_t)(run >> 32);
-        }
+        if (k <= d.nk) offs[kbase + k] = (uint32_t)run;
         run += v[t];
     }
 }
@@ -296,7 +291,7 @@ void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t
                            bufs.tiles, bufs.tiles + ntiles);
         hipLaunchKernelGGL(k_scan_top, dim3(b.ninst), dim3(1024), 0, st, b, bufs.tiles, (const uint64_t*)(bufs.tiles + ntiles));
         hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, st, b, (const uint32_t*)bufs.count,
-                           (const uint64_t*)bufs.tiles, bufs.offs, bufs.segbase);
+                           (const uint64_t*)bufs.tiles, bufs.offs);
     }
     {
         TimedLaunch tl(ctx, "msm_scatter");
@@ -305,25 +300,30 @@ void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t
     }
 }
 
+// Built-in plan (autotuned entries override it per size class).
+// c minimises n*W (bucket entries, one mixed addition each) + 3*W*2^(c-1)
+// (bucket reduction: two full additions per bucket plus its share of the run
+// weights), within the LDS histogram limit. K makes the accumulation about
+// 2^18..2^19 lanes (4 waves per SIMD on every CU), each adding exactly K
+// entries.
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     MsmPlan p;
-    int lg = 0;
-    while (((size_t)1 << (lg + 1)) <= n) ++lg;
-    int c = lg - 8;
-    if (c < 8) c = 8;
-    if (c > 16) c = 16;
-    while (c > MSM_MIN_C && ((size_t)4 << (c - 1)) > lds_bytes) --c;   // histogram must fit in LDS
+    int cmax = MSM_MAX_C;
+    while (cmax > MSM_MIN_C && ((size_t)4 << (cmax - 1)) > lds_bytes) --cmax;   // histogram must fit in LDS
+    int c = MSM_MIN_C;
+    double best = -1;
+    for (int cc = MSM_MIN_C; cc <= cmax; ++cc) {
+        double w = (double)((255 + cc - 1) / cc);
+        double cost = (double)n * w + 3.0 * w * (double)(1u << (cc - 1));
+        if (best < 0 || cost < best) { best = cost; c = cc; }
+    }
     if (force_c) c = force_c;
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.NB = 1u << (c - 1);
-    // segment length: about 2^18 lanes share the n*W additions (fewer leave
-    // SIMDs idle: one lane's chain of point additions is latency-bound), and
-    // at 2^24 points a typical bucket (~512 entries) stays one segment, so
-    // only oversized buckets pay for the segment merge
     size_t L = n * (size_t)p.W;
-    uint32_t K = 8;
-    while (K < 4096 && L / (2 * K) >= ((size_t)1 << 18)) K <<= 1;
+    uint32_t K = 4;
+    while (K < 4096 && L / (2 * (size_t)K) >= ((size_t)1 << 18)) K <<= 1;
     p.K = K;
     uint32_t g = p.NB / 512;
     if (g < 4) g = 4;

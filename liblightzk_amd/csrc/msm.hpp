@@ -15,7 +15,7 @@ struct MsmPlan {
     int c;          // window bits
     int W;          // windows, ceil(255 / c)
     uint32_t NB;    // buckets per window, 2^(c-1)
-    uint32_t K;     // entries per accumulation segment
+    uint32_t K;     // sorted entries per accumulation lane (chunk length)
     uint32_t g;     // buckets per bucket-reduction run
     uint32_t T;     // scalar tiles per window (histogram / scatter workgroups)
     uint32_t tile;  // scalars per tile
@@ -30,9 +30,9 @@ MsmPlan msm_plan_ctx(hzk_ctx* ctx, size_t n, int force_c);
 enum MsmSpace {
     SP_DIGIT = 0,   // one lane per scalar
     SP_TILEWG,      // one workgroup per (tile, window)
-    SP_KEY,         // nk + 1 entries per instance (counts, offsets, segment bases)
+    SP_KEY,         // nk + 1 entries per instance (counts, offsets, bucket sums)
     SP_SCANTILE,    // scan tiles over the key space
-    SP_SEG,         // accumulation segments (upper bound)
+    SP_CHUNK,       // accumulation chunks of K sorted entries (upper bound)
     SP_BRED,        // bucket-reduction lanes (window x run)
     SP_WIN,         // windows
     SP_ENTRY,       // n * W digit / sorted-entry slots
@@ -51,7 +51,7 @@ struct MsmBatchDev {
     const MsmInst* inst;      // [ninst]
     const uint64_t* begin;    // [SP_COUNT][ninst + 1]
     uint32_t ninst;
-    uint32_t* stats;          // per instance {entries, segments, largest bucket, 0}
+    uint32_t* stats;          // per instance {entries, chunks, largest bucket, 0}
 };
 
 // Launch the scalar-side stages (digits, LDS histograms, scans, scatter) for a
@@ -61,7 +61,6 @@ struct MsmScalarBufs {
     uint32_t* hist;      // SP_HIST
     uint32_t* count;     // SP_KEY
     uint32_t* offs;      // SP_KEY
-    uint32_t* segbase;   // SP_KEY
     uint64_t* tiles;     // 2 * SP_SCANTILE
     uint32_t* sorted;    // SP_ENTRY
 };

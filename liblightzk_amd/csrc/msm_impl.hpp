@@ -13,7 +13,8 @@
 
 namespace hzk {
 
-static constexpr int K2 = 16;   // merge fan-in
+static constexpr int K2 = 16;        // fan-in of the tree over a hot bucket's chunk partials
+static constexpr uint32_t HOT = 16;  // a bucket spanning more chunks than this uses the tree
 
 HZK_DEV uint32_t find_inst_g(const uint64_t* __restrict__ beg, uint32_t ninst, uint64_t gid) {
     uint32_t lo = 0, hi = ninst;
@@ -25,43 +26,49 @@ HZK_DEV uint32_t find_inst_g(const uint64_t* __restrict__ beg, uint32_t ninst, u
 }
 HZK_DEV const uint64_t* space_g(const MsmBatchDev& b, int s) { return b.begin + (size_t)s * (b.ninst + 1); }
 
-// index of the key whose [segbase[k], segbase[k+1]) contains s (keys 0..nk-1)
-HZK_DEV uint32_t find_key(const uint32_t* __restrict__ segbase, uint32_t nk, uint32_t s) {
-    uint32_t lo = 0, hi = nk;   // invariant: segbase[lo] <= s < segbase[hi]
+// the key whose entries [offs[k], offs[k+1]) contain entry e (offs[nk] > e)
+HZK_DEV uint32_t find_key(const uint32_t* __restrict__ offs, uint32_t nk, uint32_t e) {
+    uint32_t lo = 0, hi = nk;   // invariant: offs[lo] <= e < offs[hi]
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
-        if (segbase[mid] <= s) lo = mid; else hi = mid;
+        if (offs[mid] <= e) lo = mid; else hi = mid;
     }
     return lo;
 }
 
 // ------------------------------------------------------------------ 5. accumulate
-// lane per segment of <= K entries of one bucket: mixed additions of affine
-// bases (negated for negative digits) into an XYZZ accumulator. Persistent:
-// a fixed grid strides over the whole batch's segment space.
+// Lane per chunk of K consecutive sorted entries; a chunk crosses bucket
+// boundaries freely, so every lane does exactly K mixed additions (the last
+// chunk of an instance fewer). The running XYZZ sum is flushed at each bucket
+// end: a bucket that lies wholly inside the chunk is final and goes straight
+// to bucket[k]; the partial of a bucket that began in an earlier chunk goes
+// to carry[2c] (head), that of a bucket continuing into the next chunk to
+// carry[2c+1] (tail). Persistent grid over the whole batch's chunk space.
 template <class F>
 __global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uint32_t* __restrict__ sorted,
-                                                        const uint32_t* __restrict__ offs, const uint32_t* __restrict__ segbase,
-                                                        xyzz_t<F>* __restrict__ part, uint32_t* __restrict__ seg2key) {
-    const uint64_t* sbeg = space_g(b, SP_SEG);
-    const uint64_t total = sbeg[b.ninst];
+                                                        const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ bucket,
+                                                        xyzz_t<F>* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
+    const uint64_t total = cbeg[b.ninst];
     for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t ii = find_inst_g(sbeg, b.ninst, gid);
-        uint32_t s = (uint32_t)(gid - sbeg[ii]);
-        if (s >= b.stats[4 * ii + 1]) continue;
+        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
         const MsmInst& d = b.inst[ii];
+        const uint32_t E = b.stats[4 * ii];
+        const uint32_t e0 = (uint32_t)(gid - cbeg[ii]) * d.K;
+        if (e0 >= E) continue;
+        const uint32_t e1 = min(e0 + d.K, E);
         const uint64_t kb = space_g(b, SP_KEY)[ii];
-        const uint32_t* sb = segbase + kb;
         const uint32_t* of = offs + kb;
-        uint32_t k = find_key(sb, d.nk, s);
-        uint32_t e = of[k] + (s - sb[k]) * d.K;
-        uint32_t e1 = min(e + d.K, of[k + 1]);
+        uint32_t k = find_key(of, d.nk, e0);
+        chunk2key[gid] = k;
+        uint32_t next = of[k + 1];
+        bool before = of[k] < e0;
         const uint32_t* srt = sorted + space_g(b, SP_ENTRY)[ii];
         const affine_t<F>* bases = (const affine_t<F>*)d.bases;
         xyzz_t<F> acc = xyzz_t<F>::inf();
-        uint32_t idx = srt[e];
+        uint32_t idx = srt[e0];
         affine_t<F> P = bases[idx & 0x7fffffffu];
-        for (; e < e1; ++e) {
+        for (uint32_t e = e0; e < e1; ++e) {
             bool neg = idx >> 31;
             affine_t<F> Q = P;
             if (e + 1 < e1) {   // prefetch the next base while this one is added
@@ -70,63 +77,83 @@ __global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uin
             }
             if (neg) Q.y = Q.y.neg();
             acc.add_affine(Q);
+            if (e + 1 == next || e + 1 == e1) {
+                if (before) carry[2 * gid] = acc;
+                else if (next > e1) carry[2 * gid + 1] = acc;
+                else bucket[kb + k] = acc;
+                if (e + 1 < e1) {   // next non-empty bucket
+                    do { ++k; next = of[k + 1]; } while (next <= e + 1);
+                    before = false;
+                    acc = xyzz_t<F>::inf();
+                }
+            }
         }
-        part[gid] = acc;
-        seg2key[gid] = k;
     }
 }
 
-// ------------------------------------------------------------------ 6. merge
-// keys with 2..K2 segments: one lane per key sums them into the first slot
+// ------------------------------------------------------------------ 6. fix-up
+// hot buckets (more than HOT chunks): one level of an in-place K2-ary tree
+// over the head partials carry[2c] of chunks f+1..l (f, l: first and last
+// chunk of the bucket); afterwards carry[2(f+1)] holds their sum
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_merge_small(MsmBatchDev b, const uint32_t* __restrict__ segbase,
-                                                         xyzz_t<F>* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint32_t* __restrict__ offs,
+                                                      const uint32_t* __restrict__ chunk2key, uint32_t stride,
+                                                      xyzz_t<F>* __restrict__ carry) {
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
+    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= cbeg[b.ninst]) return;
+    uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
+    const MsmInst& d = b.inst[ii];
+    if (b.stats[4 * ii + 2] <= HOT * d.K) return;   // no bucket of this instance is hot
+    const uint32_t c = (uint32_t)(gid - cbeg[ii]);
+    if ((uint64_t)c * d.K >= b.stats[4 * ii]) return;
+    const uint32_t* of = offs + space_g(b, SP_KEY)[ii];
+    const uint32_t k = chunk2key[gid], start = of[k];
+    if (start >= c * d.K) return;   // chunk c holds no head partial
+    const uint32_t f = start / d.K, l = (of[k + 1] - 1) / d.K;
+    if (l - f <= HOT) return;
+    const uint32_t rel = c - (f + 1), span = l - f;
+    if (rel % (stride * K2) != 0) return;
+    xyzz_t<F>* base = carry + 2 * cbeg[ii];
+    xyzz_t<F> acc = base[2 * c];
+    for (int i = 1; i < K2; ++i) {
+        uint32_t r = rel + i * stride;
+        if (r >= span) break;
+        acc.add(base[2 * (c + i * stride)]);
+    }
+    base[2 * c] = acc;
+}
+
+// lane per bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l)
+template <class F>
+__global__ void __launch_bounds__(256) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
+                                                   const xyzz_t<F>* __restrict__ carry, xyzz_t<F>* __restrict__ bucket) {
     const uint64_t* kbeg = space_g(b, SP_KEY);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= kbeg[b.ninst]) return;
     uint32_t ii = find_inst_g(kbeg, b.ninst, gid);
-    uint32_t k = (uint32_t)(gid - kbeg[ii]);
-    if (k >= b.inst[ii].nk) return;
-    uint32_t b0 = segbase[gid], cnt = segbase[gid + 1] - b0;
-    if (cnt < 2 || cnt > K2) return;
-    xyzz_t<F>* p = part + space_g(b, SP_SEG)[ii] + b0;
-    xyzz_t<F> acc = p[0];
-    for (uint32_t i = 1; i < cnt; ++i) acc.add(p[i]);
-    p[0] = acc;
-}
-
-// keys with more than K2 segments (oversized buckets): one level of an
-// in-place K2-ary tree over the key's segment sums
-template <class F>
-__global__ void __launch_bounds__(256) k_msm_merge(MsmBatchDev b, const uint32_t* __restrict__ segbase,
-                                                   const uint32_t* __restrict__ seg2key, uint32_t stride,
-                                                   xyzz_t<F>* __restrict__ part) {
-    const uint64_t* sbeg = space_g(b, SP_SEG);
-    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= sbeg[b.ninst]) return;
-    uint32_t ii = find_inst_g(sbeg, b.ninst, gid);
-    uint32_t s = (uint32_t)(gid - sbeg[ii]);
-    if (s >= b.stats[4 * ii + 1]) return;
-    const uint32_t* sb = segbase + space_g(b, SP_KEY)[ii];
-    uint32_t k = seg2key[gid];
-    uint32_t b0 = sb[k], cnt = sb[k + 1] - b0, rel = s - b0;
-    if (cnt <= K2 || cnt <= stride || rel % (stride * K2) != 0) return;
-    xyzz_t<F>* p = part + sbeg[ii] + b0;
-    xyzz_t<F> acc = p[rel];
-    for (int i = 1; i < K2; ++i) {
-        uint32_t r = rel + i * stride;
-        if (r >= cnt) break;
-        acc.add(p[r]);
+    const MsmInst& d = b.inst[ii];
+    const uint32_t k = (uint32_t)(gid - kbeg[ii]);
+    if (k >= d.nk) return;
+    const uint32_t o0 = offs[gid], o1 = offs[gid + 1];
+    if (o0 == o1) return;
+    const uint32_t f = o0 / d.K, l = (o1 - 1) / d.K;
+    if (f == l) return;   // written by its chunk
+    const xyzz_t<F>* base = carry + 2 * space_g(b, SP_CHUNK)[ii];
+    xyzz_t<F> acc = base[2 * f + 1];
+    if (l - f > HOT) {
+        acc.add(base[2 * (f + 1)]);
+    } else {
+        for (uint32_t c = f + 1; c <= l; ++c) acc.add(base[2 * c]);
     }
-    p[rel] = acc;
+    bucket[gid] = acc;
 }
 
 // ------------------------------------------------------------------ 7. bucket reduction
 // lane per (window, run of g buckets): T = sum_{d in run} d * B_d
 template <class F>
-__global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xyzz_t<F>* __restrict__ part,
-                                                         const uint32_t* __restrict__ offs, const uint32_t* __restrict__ segbase,
-                                                         xyzz_t<F>* __restrict__ segsum) {
+__global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xyzz_t<F>* __restrict__ bucket,
+                                                         const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ segsum) {
     const uint64_t* rbeg = space_g(b, SP_BRED);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= rbeg[b.ninst]) return;
@@ -136,13 +163,12 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xy
     uint32_t j = t / d.G, sg = t % d.G;
     const uint64_t kb = space_g(b, SP_KEY)[ii];
     const uint32_t* of = offs + kb;
-    const uint32_t* sb = segbase + kb;
-    const xyzz_t<F>* pp = part + space_g(b, SP_SEG)[ii];
+    const xyzz_t<F>* bk = bucket + kb;
     uint32_t kbase = j * d.NB + sg * d.g;
     xyzz_t<F> run = xyzz_t<F>::inf(), acc = xyzz_t<F>::inf();
     for (int q = (int)d.g - 1; q >= 0; --q) {
         uint32_t key = kbase + q;
-        if (of[key + 1] != of[key]) run.add(pp[sb[key]]);
+        if (of[key + 1] != of[key]) run.add(bk[key]);
         acc.add(run);
     }
     // acc = sum (d - sg*g) B_d ; add (sg*g) * run
@@ -189,11 +215,10 @@ struct MsmBatchLayout {
     std::vector<uint64_t> begin;   // [SP_COUNT][ninst + 1]
     uint64_t tot[SP_COUNT];
     uint32_t max_nb = 1;
-    uint32_t merge_levels = 0;     // tree levels for oversized buckets (upper bound)
-    bool any_merge = false;
+    uint32_t tree_levels = 0;      // tree levels for hot buckets (upper bound)
     size_t bytes = 0;              // scratch needed
     // scratch offsets
-    size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_segbase, o_tiles, o_sorted, o_seg2key, o_part,
+    size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_chunk2key, o_bucket, o_carry,
         o_segsum, o_win;
 };
 
@@ -223,19 +248,18 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_TILEWG] = (uint64_t)d.T * d.W;
         sz[SP_KEY] = (uint64_t)d.nk + 1;
         sz[SP_SCANTILE] = ((uint64_t)d.nk + MSM_SCAN_TILE) / MSM_SCAN_TILE;
-        sz[SP_SEG] = (entries + d.K - 1) / d.K + d.nk;
+        sz[SP_CHUNK] = (entries + d.K - 1) / d.K;
         sz[SP_BRED] = (uint64_t)d.W * d.G;
         sz[SP_WIN] = d.W;
         sz[SP_ENTRY] = entries;
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
         for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
         if (d.NB > L.max_nb) L.max_nb = d.NB;
-        uint32_t maxseg = (d.n + d.K - 1) / d.K;   // a bucket holds at most n entries
-        if (maxseg > 1) L.any_merge = true;
+        uint64_t maxspan = (d.n + d.K - 1) / d.K + 1;   // a bucket holds at most n entries
         uint32_t lv = 0;
-        if (maxseg > (uint32_t)K2)
-            for (uint64_t stride = 1; stride < maxseg; stride *= K2) ++lv;
-        if (lv > L.merge_levels) L.merge_levels = lv;
+        if (maxspan > HOT)
+            for (uint64_t stride = 1; stride < maxspan; stride *= K2) ++lv;
+        if (lv > L.tree_levels) L.tree_levels = lv;
     }
     for (int s = 0; s < SP_COUNT; ++s) L.tot[s] = L.begin[(size_t)s * (count + 1) + count];
     size_t off = 0;
@@ -247,11 +271,11 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_hist = take(L.tot[SP_HIST] * 4);
     L.o_count = take(L.tot[SP_KEY] * 4);
     L.o_offs = take(L.tot[SP_KEY] * 4);
-    L.o_segbase = take(L.tot[SP_KEY] * 4);
     L.o_tiles = take(2 * L.tot[SP_SCANTILE] * 8);
     L.o_sorted = take(L.tot[SP_ENTRY] * 4);
-    L.o_seg2key = take(L.tot[SP_SEG] * 4);
-    L.o_part = take(L.tot[SP_SEG] * sizeof(xyzz_t<F>));
+    L.o_chunk2key = take(L.tot[SP_CHUNK] * 4);
+    L.o_bucket = take(L.tot[SP_KEY] * sizeof(xyzz_t<F>));
+    L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
     L.o_segsum = take(L.tot[SP_BRED] * sizeof(xyzz_t<F>));
     L.o_win = take(L.tot[SP_WIN] * sizeof(xyzz_t<F>));
     L.bytes = off + 256;
@@ -272,11 +296,11 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     sb.hist = (uint32_t*)(base + L.o_hist);
     sb.count = (uint32_t*)(base + L.o_count);
     sb.offs = (uint32_t*)(base + L.o_offs);
-    sb.segbase = (uint32_t*)(base + L.o_segbase);
     sb.tiles = (uint64_t*)(base + L.o_tiles);
     sb.sorted = (uint32_t*)(base + L.o_sorted);
-    uint32_t* seg2key = (uint32_t*)(base + L.o_seg2key);
-    xyzz_t<F>* part = (xyzz_t<F>*)(base + L.o_part);
+    uint32_t* chunk2key = (uint32_t*)(base + L.o_chunk2key);
+    xyzz_t<F>* bucket = (xyzz_t<F>*)(base + L.o_bucket);
+    xyzz_t<F>* carry = (xyzz_t<F>*)(base + L.o_carry);
     xyzz_t<F>* segsum = (xyzz_t<F>*)(base + L.o_segsum);
     xyzz_t<F>* win = (xyzz_t<F>*)(base + L.o_win);
     hipStream_t st = ctx->stream;
@@ -284,24 +308,24 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     msm_launch_scalar_stages(ctx, b, L.tot, L.max_nb, sb);
     {
         TimedLaunch tl(ctx, "msm_accumulate");
-        uint64_t blocks = (L.tot[SP_SEG] + 255) / 256;
+        uint64_t blocks = (L.tot[SP_CHUNK] + 255) / 256;
         uint64_t cap = (uint64_t)ctx->num_cus * 16;   // persistent grid: 16 waves per CU
         hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                           (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, (const uint32_t*)sb.segbase, part, seg2key);
+                           (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
     }
-    if (L.any_merge) {
-        TimedLaunch tl(ctx, "msm_merge");
-        hipLaunchKernelGGL(k_msm_merge_small<F>, dim3(blocks_for(L.tot[SP_KEY], 256)), dim3(256), 0, st, b,
-                           (const uint32_t*)sb.segbase, part);
+    {
+        TimedLaunch tl(ctx, "msm_fixup");
         uint32_t stride = 1;
-        for (uint32_t lv = 0; lv < L.merge_levels; ++lv, stride *= K2)
-            hipLaunchKernelGGL(k_msm_merge<F>, dim3(blocks_for(L.tot[SP_SEG], 256)), dim3(256), 0, st, b,
-                               (const uint32_t*)sb.segbase, (const uint32_t*)seg2key, stride, part);
+        for (uint32_t lv = 0; lv < L.tree_levels; ++lv, stride *= K2)
+            hipLaunchKernelGGL(k_msm_hot_tree<F>, dim3(blocks_for(L.tot[SP_CHUNK], 256)), dim3(256), 0, st, b,
+                               (const uint32_t*)sb.offs, (const uint32_t*)chunk2key, stride, carry);
+        hipLaunchKernelGGL(k_msm_fixup<F>, dim3(blocks_for(L.tot[SP_KEY], 256)), dim3(256), 0, st, b, (const uint32_t*)sb.offs,
+                           (const xyzz_t<F>*)carry, bucket);
     }
     {
         TimedLaunch tl(ctx, "msm_bucket_reduce");
         hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b,
-                           (const xyzz_t<F>*)part, (const uint32_t*)sb.offs, (const uint32_t*)sb.segbase, segsum);
+                           (const xyzz_t<F>*)bucket, (const uint32_t*)sb.offs, segsum);
         hipLaunchKernelGGL(k_window_sum<F>, dim3((unsigned)L.tot[SP_WIN]), dim3(256), 0, st, b, (const xyzz_t<F>*)segsum, win);
     }
 }
@@ -328,6 +352,9 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     }
     msm_prepare();
     MsmBatchLayout L = msm_layout<F>(ctx, jobs, count);
+    for (size_t i = 0; i < count; ++i)   // bucket offsets are 32-bit
+        HZK_CHECK(ctx, (uint64_t)L.inst[i].n * L.inst[i].W <= 0xffff0000ull, HZK_ERR_UNSUPPORTED,
+                  "msm: n=%u with %u windows exceeds 2^32 bucket entries", L.inst[i].n, L.inst[i].W);
     HZK_CHECK(ctx, (size_t)L.max_nb * 4 <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
               "msm: a %u-bucket window needs %u B of LDS per workgroup, device allows %zu", L.max_nb, L.max_nb * 4,
               ctx->lds_per_block);
