@@ -217,6 +217,7 @@ def main():
     d_out = ctx.alloc(96)
 
     peak_mad = ctx.roofline_probe("mad")
+    hbm_copy = ctx.roofline_probe("copy")
     for _ in range(args.warmup):
         ctx.msm_g1(d_s, d_b, n, d_out)
     ctx.sync()
@@ -249,6 +250,9 @@ def main():
         "work_per_launch": work,
         "avg_launch_ms": acc_avg_s * 1e3,
         "msm_intmul_frac": (w_msm(n) / (el / args.steps)) / peak_mad if peak_mad else None,
+        "algorithmic_bytes_per_step": n * 96,
+        "hbm_frac": n * 96 / (el / args.steps) / HBM_PEAK,
+        "hbm_copy_measured_GBps": hbm_copy / 1e9,
     }
     out = {
         "metric": METRIC,

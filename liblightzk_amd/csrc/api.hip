@@ -106,6 +106,31 @@ static void drain_timing(hzk_ctx* ctx) {
     ctx->pending.clear();
 }
 
+// the auxiliary context (second stream and scratch) used to overlap a G2 batch
+// with a G1 batch; inherits the timing switch
+int ctx_aux(hzk_ctx* ctx, hzk_ctx** out) {
+    if (!ctx->aux) {
+        int rc = hzk_ctx_create(ctx->device, nullptr, &ctx->aux);
+        if (rc) return set_error(ctx, rc, "auxiliary context: creation failed");
+        std::memcpy(ctx->aux->tune, ctx->tune, sizeof ctx->tune);
+    }
+    ctx->aux->timing = ctx->timing;
+    ctx->aux->fail_alloc_after = -1;
+    *out = ctx->aux;
+    return 0;
+}
+
+// fold the auxiliary context's kernel times into ctx's totals
+void timing_merge(hzk_ctx* ctx, hzk_ctx* from) {
+    drain_timing(from);
+    for (auto& kv : from->totals) {
+        auto& t = ctx->totals[kv.first];
+        t.first += kv.second.first;
+        t.second += kv.second.second;
+    }
+    from->totals.clear();
+}
+
 TimedLaunch::TimedLaunch(hzk_ctx* c, const char* name) : ctx(c) {
     if (!ctx->timing) return;
     hipEvent_t a = take_event(ctx);
@@ -176,6 +201,7 @@ int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
 
 void hzk_ctx_destroy(hzk_ctx* ctx) {
     if (!ctx) return;
+    hzk_ctx_destroy(ctx->aux);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     drain_timing(ctx);

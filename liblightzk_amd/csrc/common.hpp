@@ -63,6 +63,8 @@ struct hzk_ctx {
     // NTT twiddle / coset tables for the last transform size (ntt.hip)
     void* ntt_cache = nullptr;
     hzk_tune_entry tune[32];
+    // second context (own stream and scratch) for work overlapped with this one's
+    hzk_ctx* aux = nullptr;
     // autotune overrides (0 = automatic)
     int force_window = 0;
     int force_seg = 0;
@@ -97,6 +99,8 @@ struct Arena {
 };
 // Make sure ctx->scratch holds at least `bytes`; returns 0 or an error code.
 int ensure_scratch(hzk_ctx* ctx, size_t bytes);
+int ctx_aux(hzk_ctx* ctx, hzk_ctx** out);
+void timing_merge(hzk_ctx* ctx, hzk_ctx* from);
 // Pinned host staging of at least `bytes`: *buf is writable now; call
 // stage_fence(ctx, slot) after enqueueing the copies that read it.
 int stage_acquire(hzk_ctx* ctx, size_t bytes, char** buf, int* slot);

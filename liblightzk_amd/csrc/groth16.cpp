@@ -24,6 +24,7 @@
 // Streams: 2 circuit structure, 3 witness inputs, 4 toxic waste
 // (tau, alpha, beta, gamma, delta), 5 prover randomness (r, s).
 #include <omp.h>
+#include <thread>
 #include <vector>
 #include "common.hpp"
 #include "host_ec.hpp"
@@ -250,8 +251,27 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
         j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0});
         j2.push_back({reqs[i].z, g->B2, N1, r + 192, 0});
     }
-    int rc = msm_batch_g1(ctx, j1.data(), j1.size(), 1, graph);
-    if (!rc) rc = msm_batch_g2(ctx, j2.data(), j2.size(), 1, graph);
+    // the G2 batch runs on the auxiliary context's stream, concurrently with
+    // the G1 batch: the latency-bound tails of one (bucket reduction, window
+    // sums, host combine) overlap the throughput-bound accumulation of the other
+    hzk_ctx* aux = nullptr;
+    int rc = ctx_aux(ctx, &aux);
+    if (rc) return rc;
+    hipEvent_t ready;
+    HZK_HIP(ctx, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ready, ctx->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(aux->stream, ready, 0);
+    (void)hipEventDestroy(ready);
+    HZK_HIP(ctx, e);
+    int rc2 = 0;
+    std::thread t2([&] {
+        (void)hipSetDevice(aux->device);
+        rc2 = msm_batch_g2(aux, j2.data(), j2.size(), 1, graph);
+    });
+    rc = msm_batch_g1(ctx, j1.data(), j1.size(), 1, graph);
+    t2.join();
+    if (ctx->timing) timing_merge(ctx, aux);
+    if (!rc && rc2) return set_error(ctx, rc2, "%s", hzk_last_error(aux));
     return rc;
 }
 

@@ -33,6 +33,7 @@
 // one; a batch of uneven sizes fills the GPU where one small MSM cannot.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include "common.hpp"
 #include "curve.hpp"
 #include "msm.hpp"
@@ -257,12 +258,12 @@ __global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint3
 
 // ------------------------------------------------------------------ host side
 void msm_prepare() {
-    static bool done = false;
-    if (done) return;
-    done = true;
-    int bytes = (1 << (MSM_MAX_C - 1)) * 4;
-    (void)hipFuncSetAttribute((const void*)k_msm_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    (void)hipFuncSetAttribute((const void*)k_msm_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int bytes = (1 << (MSM_MAX_C - 1)) * 4;
+        (void)hipFuncSetAttribute((const void*)k_msm_hist, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute((const void*)k_msm_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    });
 }
 
 static unsigned grid_for(uint64_t n, unsigned threads, unsigned cap) {
@@ -322,7 +323,7 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     p.W = (255 + c - 1) / c;
     p.NB = 1u << (c - 1);
     size_t L = n * (size_t)p.W;
-    uint32_t K = 4;
+    uint32_t K = 4;   // multiple of 4: the accumulation reads indices in groups of four
     while (K < 4096 && L / (2 * (size_t)K) >= ((size_t)1 << 18)) K <<= 1;
     p.K = K;
     uint32_t g = p.NB / 512;

@@ -63,16 +63,22 @@ __global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uin
         chunk2key[gid] = k;
         uint32_t next = of[k + 1];
         bool before = of[k] < e0;
-        const uint32_t* srt = sorted + space_g(b, SP_ENTRY)[ii];
+        // sorted indices come in 16-byte groups (instance ranges are 256-B
+        // aligned and K is a multiple of 4): one load per four entries keeps
+        // each lane's 128-byte index line from being fetched again and again
+        const uint4* srt4 = (const uint4*)(sorted + space_g(b, SP_ENTRY)[ii]);
         const affine_t<F>* bases = (const affine_t<F>*)d.bases;
         xyzz_t<F> acc = xyzz_t<F>::inf();
-        uint32_t idx = srt[e0];
+        uint4 q = srt4[e0 >> 2];
+        uint32_t idx = q.x;
         affine_t<F> P = bases[idx & 0x7fffffffu];
         for (uint32_t e = e0; e < e1; ++e) {
             bool neg = idx >> 31;
             affine_t<F> Q = P;
             if (e + 1 < e1) {   // prefetch the next base while this one is added
-                idx = srt[e + 1];
+                const uint32_t t = (e + 1) & 3;
+                if (t == 0) q = srt4[(e + 1) >> 2];
+                idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
                 P = bases[idx & 0x7fffffffu];
             }
             if (neg) Q.y = Q.y.neg();
@@ -251,7 +257,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_CHUNK] = (entries + d.K - 1) / d.K;
         sz[SP_BRED] = (uint64_t)d.W * d.G;
         sz[SP_WIN] = d.W;
-        sz[SP_ENTRY] = entries;
+        sz[SP_ENTRY] = (entries + 63) & ~(uint64_t)63;   // 256-B aligned per instance (vector index loads)
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
         for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
         if (d.NB > L.max_nb) L.max_nb = d.NB;

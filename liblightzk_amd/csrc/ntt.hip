@@ -16,6 +16,7 @@
 // (R = 9, then the last 11 stages, whose rows are contiguous). One more pass
 // applies the bit-reverse permutation fused with the scale factor.
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "common.hpp"
@@ -260,11 +261,13 @@ static int ensure_tables(hzk_ctx* ctx, int logn, NttTables** out) {
 }
 
 static void set_ntt_lds() {
-    static bool done = false;
-    if (done) return;
-    done = true;
-    (void)hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (1 << NTT_TILE_BITS) * 32);
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (1 << NTT_TILE_BITS) * 32);
+        (void)hipFuncSetAttribute((const void*)k_ntt_pass_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (1 << NTT_TILE_BITS) * 32);
+    });
 }
 
 // In-place transform with twiddle table tw (w^e, e < n/2), optional device factor.

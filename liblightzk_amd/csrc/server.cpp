@@ -186,7 +186,8 @@ int stage_msm(hzk_server* s, std::vector<Request*>& rs) {
     for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
     int rc = g16_msm_batch(s->ctx_m, reqs.data(), reqs.size(), 0);
     if (rc) return rc;
-    for (size_t i = 0; i < rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, rs[i]->proof);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, rs[i]->proof);
     return 0;
 }
 
