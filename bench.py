@@ -267,7 +267,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded Fr scalars; bases k_i*G1 generated on the GPU)",
-        "config": {"workload": f"g1_msm_2^{args.log_n}_uniform", "n": n, "window_bits": st["c"],
+        "config": {"workload": f"g1_msm_2^{args.log_n}_uniform", "n": n, "plan": "built-in autotuned table (liblightzk_amd/tune/mi355x.json)",
+                   "window_bits": st["c"], "chunk_K": ctx.msm_tune_get(1, args.log_n)[1],
                    "windows": st["windows"], "bucket_entries": st["entries"], "parallelism": "single-gpu"},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in rep.items()},
         "roofline": roofline,

@@ -239,6 +239,21 @@ size_t hzk_timing_report(hzk_ctx* ctx, char* buf, size_t cap);
  * 3 HBM copy bytes per second (read + write). */
 int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value);
 
+/* ---- MSM autotuning (no counterpart in the reference, which hard-wires
+ * c = 12 and its launch geometry, include/operator/msm.cuh:121, :225, :245).
+ * A plan per group (1 = G1, 2 = G2) and size class log_n = floor(log2 n):
+ * window bits c, accumulation chunk length K (entries per lane, a multiple of
+ * 4) and bucket-reduction run length g (a power of two <= 2^(c-1)). Every MSM
+ * on the context, single or batched, uses the class's plan; results do not
+ * depend on the plan. c = 0 clears the entry (built-in heuristic). */
+int hzk_msm_tune_set(hzk_ctx* ctx, int group, int log_n, int c, uint32_t K, uint32_t g);
+/* the plan an MSM of 2^log_n points would use now */
+int hzk_msm_tune_get(hzk_ctx* ctx, int group, int log_n, int* c, uint32_t* K, uint32_t* g);
+/* time candidate plans for classes log_min..log_max on seeded synthetic
+ * inputs (coordinate descent over c, K, g; at most budget_ms per class) and
+ * store the fastest in the context's table */
+int hzk_msm_autotune(hzk_ctx* ctx, int group, int log_min, int log_max, double budget_ms);
+
 /* ---- tuning / test hooks ---- */
 /* force MSM window bits and bucket-reduction segment size (0 = automatic) */
 int hzk_set_tuning(hzk_ctx* ctx, int window_bits, int segment);

@@ -172,6 +172,18 @@ class Context:
     def set_tuning(self, window_bits=0, segment=0):
         self._check(load().hzk_set_tuning(self.handle, window_bits, segment))
 
+    def msm_tune_set(self, group, log_n, c, K=0, g=0):
+        self._check(load().hzk_msm_tune_set(self.handle, group, log_n, c, K, g))
+
+    def msm_tune_get(self, group, log_n):
+        c, K, g = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(load().hzk_msm_tune_get(self.handle, group, log_n, ctypes.byref(c), ctypes.byref(K), ctypes.byref(g)))
+        return c.value, K.value, g.value
+
+    def msm_autotune(self, group, log_min, log_max, budget_ms=2000.0):
+        self._check(load().hzk_msm_autotune(self.handle, group, log_min, log_max, float(budget_ms)))
+        return {lg: self.msm_tune_get(group, lg) for lg in range(log_min, log_max + 1)}
+
     def inject_alloc_failure(self, after_n):
         self._check(load().hzk_inject_alloc_failure(self.handle, after_n))
 

@@ -62,7 +62,7 @@ struct hzk_ctx {
     uint32_t last_msm[5] = {0, 0, 0, 0, 0};
     // NTT twiddle / coset tables for the last transform size (ntt.hip)
     void* ntt_cache = nullptr;
-    hzk_tune_entry tune[32];
+    hzk_tune_entry tune[2][32];   // [G1, G2][log2 n]
     // second context (own stream and scratch) for work overlapped with this one's
     hzk_ctx* aux = nullptr;
     // autotune overrides (0 = automatic)

@@ -340,14 +340,24 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     return p;
 }
 
-MsmPlan msm_plan_ctx(hzk_ctx* ctx, size_t n, int force_c) {
+// plans measured by the autotuner on an MI355X (liblightzk_amd/tune/mi355x.json)
+static const hzk_tune_entry BUILTIN_PLANS[2][32] = {
+#include "tune_table.inc"
+};
+
+// plan for an MSM of n points: the context's tuned entry for the size class,
+// else the built-in measured plan, else the heuristic; a caller-forced window
+// size (or hzk_set_tuning) bypasses both tables
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c) {
     MsmPlan p = msm_plan(n, force_c ? force_c : ctx->force_window, ctx->force_seg, ctx->lds_per_block);
     if (!force_c && !ctx->force_window) {
         int cls = 0;
         while (((size_t)1 << (cls + 1)) <= n) ++cls;
-        const hzk_tune_entry& t = ctx->tune[cls < 32 ? cls : 31];
-        if (t.c) {
-            MsmPlan q = msm_plan(n, t.c, t.g ? t.g : ctx->force_seg, ctx->lds_per_block);
+        cls = cls < 32 ? cls : 31;
+        hzk_tune_entry t = ctx->tune[g2 ? 1 : 0][cls];
+        if (!t.c && !ctx->force_seg) t = BUILTIN_PLANS[g2 ? 1 : 0][cls];
+        if (t.c && ((size_t)4 << (t.c - 1)) <= ctx->lds_per_block) {
+            MsmPlan q = msm_plan(n, t.c, t.g ? (int)t.g : ctx->force_seg, ctx->lds_per_block);
             if (t.K) q.K = t.K;
             p = q;
         }

@@ -23,7 +23,7 @@ struct MsmPlan {
 // Plan for n points: autotuned entry for the size class if one is loaded,
 // else the built-in heuristic; force_c / force_g override (0 = automatic).
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes);
-MsmPlan msm_plan_ctx(hzk_ctx* ctx, size_t n, int force_c);
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c);
 
 // Work spaces of the batched stages; each instance owns a contiguous range
 // [begin[s][i], begin[s][i+1]) of every space.

@@ -234,7 +234,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.inst.resize(count);
     L.begin.assign((size_t)SP_COUNT * (count + 1), 0);
     for (size_t i = 0; i < count; ++i) {
-        MsmPlan p = msm_plan_ctx(ctx, jobs[i].n, (int)jobs[i].win_siz);
+        MsmPlan p = msm_plan_ctx(ctx, sizeof(F) == 32 ? 0 : 1, jobs[i].n, (int)jobs[i].win_siz);
         MsmInst& d = L.inst[i];
         d.scalars = jobs[i].scalars;
         d.bases = jobs[i].bases;
