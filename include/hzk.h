@@ -200,6 +200,10 @@ int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count
  * function of (circuit, z, r|s) alone: the same alone or in any mix. */
 typedef struct hzk_server hzk_server;
 int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hzk_server** out);
+/* number of pipelines (worker thread + two streams + MSM scratch each) sharing
+ * the admission queue, 1..8, before hzk_server_start; default 2. In mode 1
+ * only the first pipeline proves. */
+int hzk_server_set_pipelines(hzk_server* s, int n);
 void hzk_server_destroy(hzk_server* s);
 const char* hzk_server_last_error(const hzk_server* s);
 /* set up a circuit (as hzk_g16_setup) before hzk_server_start */

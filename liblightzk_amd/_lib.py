@@ -73,6 +73,7 @@ def load():
         "hzk_g16_key_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
         "hzk_server_create": (i32, [i32, sz, i32, i32, ctypes.POINTER(vp)]),
         "hzk_server_destroy": (None, [vp]),
+        "hzk_server_set_pipelines": (i32, [vp, i32]),
         "hzk_server_last_error": (ctypes.c_char_p, [vp]),
         "hzk_server_add_circuit": (i32, [vp, ctypes.c_uint64, i32, i32, ctypes.POINTER(i32)]),
         "hzk_server_circuit": (i32, [vp, i32, ctypes.POINTER(vp)]),

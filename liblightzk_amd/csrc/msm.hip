@@ -41,6 +41,7 @@
 namespace hzk {
 
 static constexpr uint32_t DIGIT_NEG = 0x80000000u;
+static constexpr uint32_t MSM_MAX_FINE = (1u << (MSM_MAX_C - 1)) / MSM_COARSE;   // buckets per coarse group at most
 static constexpr int SCAN_ITEMS = 4;
 static constexpr int SCAN_THREADS = 256;
 static_assert(SCAN_ITEMS * SCAN_THREADS == MSM_SCAN_TILE, "scan tile");
@@ -236,6 +237,7 @@ __global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint3
     const uint64_t* beg = space(b, SP_TILEWG);
     uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
+    if (d.NB >= MSM_TWO_PASS_NB) return;   // scattered in two passes below
     uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     uint32_t t = local % d.T, j = local / d.T;
     const uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
@@ -253,6 +255,150 @@ __global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint3
             uint32_t pos = atomicAdd(&cur[m - 1], 1u);
             out[pos] = i | (dg & DIGIT_NEG);
         }
+    }
+}
+
+// ------------------------------------------------------------------ 4'. two-pass scatter
+// With 2^(c-1) >= 2048 buckets a single scatter writes each 4-byte entry to a
+// different bucket range: on the MI355X that became ~8x the entry bytes in HBM
+// writes (every write a separate partially-filled sector). The two passes
+// below write coalesced runs instead: pass 1 partitions each tile's entries by
+// coarse key (the top 8 bits of the bucket index, 256 groups per window) into
+// an 8-byte staging array laid out exactly like the final bucket ranges;
+// pass 2 sorts each coarse group by its low bits. In both, a round of
+// SC_ROUND entries is counting-sorted in LDS first, so consecutive lanes write
+// consecutive addresses of one group.
+static constexpr uint32_t SC_ROUND = 4096;
+static constexpr uint32_t SC_THREADS = 1024;
+static constexpr uint32_t SC_PER = SC_ROUND / SC_THREADS;
+static constexpr uint32_t SC_NONE = 0xffffffffu;
+
+// exclusive scan of n <= 1024 LDS counters (one thread per counter)
+HZK_DEV void lds_exclusive_scan(const uint32_t* cnt, uint32_t* out, uint32_t* tmp, uint32_t n) {
+    const uint32_t t = threadIdx.x;
+    if (t < n) tmp[t] = cnt[t];
+    __syncthreads();
+    for (uint32_t off = 1; off < n; off <<= 1) {
+        uint32_t v = (t < n && t >= off) ? tmp[t - off] : 0;
+        __syncthreads();
+        if (t < n) tmp[t] += v;
+        __syncthreads();
+    }
+    if (t < n) out[t] = tmp[t] - cnt[t];
+    __syncthreads();
+}
+
+// pass 1: workgroup per (tile, window) of a two-pass instance
+__global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_coarse(MsmBatchDev b, const uint32_t* __restrict__ digits,
+                                                                   const uint32_t* __restrict__ hist, const uint32_t* __restrict__ offs,
+                                                                   uint2* __restrict__ coarse) {
+    __shared__ uint32_t cur[MSM_COARSE], cnt[MSM_COARSE], lstart[MSM_COARSE], tmp[MSM_COARSE];
+    __shared__ uint2 stage[SC_ROUND];
+    __shared__ uint8_t skey[SC_ROUND];
+    const uint64_t* beg = space(b, SP_TILEWG);
+    const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    if (d.NB < MSM_TWO_PASS_NB) return;
+    const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
+    const uint32_t t = local % d.T, j = local / d.T;
+    const uint32_t FB = d.NB / MSM_COARSE, fshift = __ffs(FB) - 1;
+    const uint32_t tid = threadIdx.x;
+    // cursor of coarse group h: start of its first bucket + this tile's prefix over its buckets
+    if (tid < MSM_COARSE) {
+        const uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB + (size_t)tid * FB;
+        uint32_t acc = offs[space(b, SP_KEY)[ii] + (size_t)j * d.NB + (size_t)tid * FB];
+        for (uint32_t q = 0; q < FB; ++q) acc += row[q];
+        cur[tid] = acc;
+    }
+    const uint32_t i0 = t * d.tile, i1 = min(d.n, i0 + d.tile);
+    const uint64_t ebase = space(b, SP_ENTRY)[ii];
+    const uint32_t* dj = digits + ebase + (size_t)j * d.n;
+    uint2* out = coarse + ebase;
+    for (uint32_t base = i0; base < i1; base += SC_ROUND) {
+        if (tid < MSM_COARSE) cnt[tid] = 0;
+        __syncthreads();
+        uint32_t val[SC_PER], key[SC_PER], rank[SC_PER];
+        for (uint32_t k = 0; k < SC_PER; ++k) {
+            const uint32_t i = base + tid + k * SC_THREADS;
+            rank[k] = SC_NONE;
+            if (i < i1) {
+                const uint32_t dg = dj[i], m = dg & ~DIGIT_NEG;
+                if (m) {
+                    key[k] = m - 1;
+                    val[k] = i | (dg & DIGIT_NEG);
+                    rank[k] = atomicAdd(&cnt[(m - 1) >> fshift], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        lds_exclusive_scan(cnt, lstart, tmp, MSM_COARSE);
+        for (uint32_t k = 0; k < SC_PER; ++k) {
+            if (rank[k] == SC_NONE) continue;
+            const uint32_t h = key[k] >> fshift, p = lstart[h] + rank[k];
+            stage[p] = make_uint2(val[k], key[k] & (FB - 1));
+            skey[p] = (uint8_t)h;
+        }
+        __syncthreads();
+        const uint32_t total = lstart[MSM_COARSE - 1] + cnt[MSM_COARSE - 1];
+        for (uint32_t p = tid; p < total; p += SC_THREADS) {
+            const uint32_t h = skey[p];
+            out[cur[h] + (p - lstart[h])] = stage[p];
+        }
+        __syncthreads();
+        if (tid < MSM_COARSE) cur[tid] += cnt[tid];
+        __syncthreads();
+    }
+}
+
+// pass 2: workgroup per (window, coarse group): bucket order inside the group
+__global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_fine(MsmBatchDev b, const uint2* __restrict__ coarse,
+                                                                 const uint32_t* __restrict__ offs, uint32_t* __restrict__ sorted) {
+    __shared__ uint32_t cur[MSM_MAX_FINE], cnt[MSM_MAX_FINE], lstart[MSM_MAX_FINE], tmp[MSM_MAX_FINE];
+    __shared__ uint32_t stage[SC_ROUND];
+    __shared__ uint8_t skey[SC_ROUND];
+    const uint64_t* beg = space(b, SP_CGROUP);
+    const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
+    const MsmInst& d = b.inst[ii];
+    if (d.NB < MSM_TWO_PASS_NB) return;
+    const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
+    const uint32_t FB = d.NB / MSM_COARSE;
+    const uint32_t* of = offs + space(b, SP_KEY)[ii] + (size_t)local * FB;   // local = j * 256 + h
+    const uint32_t tid = threadIdx.x;
+    const uint32_t g0 = of[0], g1 = of[FB];
+    if (tid < FB) cur[tid] = of[tid];
+    const uint64_t ebase = space(b, SP_ENTRY)[ii];
+    const uint2* in = coarse + ebase;
+    uint32_t* out = sorted + ebase;
+    for (uint32_t base = g0; base < g1; base += SC_ROUND) {
+        if (tid < FB) cnt[tid] = 0;
+        __syncthreads();
+        uint2 v[SC_PER];
+        uint32_t rank[SC_PER];
+        for (uint32_t k = 0; k < SC_PER; ++k) {
+            const uint32_t e = base + tid + k * SC_THREADS;
+            rank[k] = SC_NONE;
+            if (e < g1) {
+                v[k] = in[e];
+                rank[k] = atomicAdd(&cnt[v[k].y], 1u);
+            }
+        }
+        __syncthreads();
+        lds_exclusive_scan(cnt, lstart, tmp, FB);
+        for (uint32_t k = 0; k < SC_PER; ++k) {
+            if (rank[k] == SC_NONE) continue;
+            const uint32_t p = lstart[v[k].y] + rank[k];
+            stage[p] = v[k].x;
+            skey[p] = (uint8_t)v[k].y;
+        }
+        __syncthreads();
+        const uint32_t total = min(SC_ROUND, g1 - base);
+        for (uint32_t p = tid; p < total; p += SC_THREADS) {
+            const uint32_t f = skey[p];
+            out[cur[f] + (p - lstart[f])] = stage[p];
+        }
+        __syncthreads();
+        if (tid < FB) cur[tid] += cnt[tid];
+        __syncthreads();
     }
 }
 
@@ -298,6 +444,12 @@ void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t
         TimedLaunch tl(ctx, "msm_scatter");
         hipLaunchKernelGGL(k_msm_scatter, dim3((unsigned)tot[SP_TILEWG]), dim3(1024), max_nb * 4, st, b,
                            (const uint32_t*)bufs.digits, (const uint32_t*)bufs.hist, (const uint32_t*)bufs.offs, bufs.sorted);
+        if (bufs.coarse) {
+            hipLaunchKernelGGL(k_msm_scatter_coarse, dim3((unsigned)tot[SP_TILEWG]), dim3(SC_THREADS), 0, st, b,
+                               (const uint32_t*)bufs.digits, (const uint32_t*)bufs.hist, (const uint32_t*)bufs.offs, bufs.coarse);
+            hipLaunchKernelGGL(k_msm_scatter_fine, dim3((unsigned)tot[SP_CGROUP]), dim3(SC_THREADS), 0, st, b,
+                               (const uint2*)bufs.coarse, (const uint32_t*)bufs.offs, bufs.sorted);
+        }
     }
 }
 

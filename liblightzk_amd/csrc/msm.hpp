@@ -3,6 +3,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <hip/hip_runtime.h>
 
 struct hzk_ctx;
 
@@ -37,6 +38,7 @@ enum MsmSpace {
     SP_WIN,         // windows
     SP_ENTRY,       // n * W digit / sorted-entry slots
     SP_HIST,        // T * nk per-tile histogram counters
+    SP_CGROUP,      // W * 256 coarse key groups (two-pass scatter)
     SP_COUNT
 };
 
@@ -63,7 +65,11 @@ struct MsmScalarBufs {
     uint32_t* offs;      // SP_KEY
     uint64_t* tiles;     // 2 * SP_SCANTILE
     uint32_t* sorted;    // SP_ENTRY
+    uint2* coarse;       // SP_ENTRY 8-byte entries, or null when no instance uses the two-pass scatter
 };
+// instances with at least this many buckets per window scatter in two passes
+constexpr uint32_t MSM_TWO_PASS_NB = 2048;
+constexpr uint32_t MSM_COARSE = 256;   // coarse key groups per window
 void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t* host_totals, uint32_t max_nb,
                               const MsmScalarBufs& bufs);
 constexpr uint32_t MSM_SCAN_TILE = 1024;

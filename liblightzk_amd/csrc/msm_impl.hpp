@@ -222,10 +222,11 @@ struct MsmBatchLayout {
     uint64_t tot[SP_COUNT];
     uint32_t max_nb = 1;
     uint32_t tree_levels = 0;      // tree levels for hot buckets (upper bound)
+    bool two_pass = false;         // some instance scatters in two passes
     size_t bytes = 0;              // scratch needed
     // scratch offsets
-    size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_chunk2key, o_bucket, o_carry,
-        o_segsum, o_win;
+    size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_coarse, o_chunk2key, o_bucket,
+        o_carry, o_segsum, o_win;
 };
 
 template <class F>
@@ -259,6 +260,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_WIN] = d.W;
         sz[SP_ENTRY] = (entries + 63) & ~(uint64_t)63;   // 256-B aligned per instance (vector index loads)
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
+        sz[SP_CGROUP] = d.NB >= MSM_TWO_PASS_NB ? (uint64_t)d.W * MSM_COARSE : 0;
+        if (d.NB >= MSM_TWO_PASS_NB) L.two_pass = true;
         for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
         if (d.NB > L.max_nb) L.max_nb = d.NB;
         uint64_t maxspan = (d.n + d.K - 1) / d.K + 1;   // a bucket holds at most n entries
@@ -279,6 +282,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_offs = take(L.tot[SP_KEY] * 4);
     L.o_tiles = take(2 * L.tot[SP_SCANTILE] * 8);
     L.o_sorted = take(L.tot[SP_ENTRY] * 4);
+    L.o_coarse = take(L.two_pass ? L.tot[SP_ENTRY] * 8 : 0);
     L.o_chunk2key = take(L.tot[SP_CHUNK] * 4);
     L.o_bucket = take(L.tot[SP_KEY] * sizeof(xyzz_t<F>));
     L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
@@ -304,6 +308,7 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     sb.offs = (uint32_t*)(base + L.o_offs);
     sb.tiles = (uint64_t*)(base + L.o_tiles);
     sb.sorted = (uint32_t*)(base + L.o_sorted);
+    sb.coarse = L.two_pass ? (uint2*)(base + L.o_coarse) : nullptr;
     uint32_t* chunk2key = (uint32_t*)(base + L.o_chunk2key);
     xyzz_t<F>* bucket = (xyzz_t<F>*)(base + L.o_bucket);
     xyzz_t<F>* carry = (xyzz_t<F>*)(base + L.o_carry);

@@ -4,7 +4,10 @@
 //   * Several circuits are set up once; their proving keys stay in HBM.
 //   * Clients submit (circuit, witness, r|s) from any thread; each request is
 //     stamped on arrival and on completion (latency = completion - arrival).
-//   * A worker runs steps. Each step admits pending requests in arrival order
+//   * Several pipelines (a worker thread with its own two streams and MSM
+//     scratch) share one admission queue, so one pipeline's host phases
+//     (Horner combine, assembly) and latency-bound kernels overlap another's
+//     accumulation. Each pipeline runs steps. A step admits pending requests in arrival order
 //     while their device footprint fits the memory budget (a request that can
 //     never fit fails; one that does not fit now waits), then runs two stages
 //     concurrently on two streams:
@@ -60,13 +63,19 @@ struct BufSet {
 
 }  // namespace
 
+struct Pipeline {
+    int index = 0;
+    hzk_ctx* ctx_h = nullptr;   // stage 1 stream
+    hzk_ctx* ctx_m = nullptr;   // stage 2 stream (and its auxiliary context for G2)
+    std::thread th;
+};
+
 struct hzk_server {
     int device = 0;
     int mode = 0;
     int max_batch = 64;
     size_t budget = 0, in_use = 0;
-    hzk_ctx* ctx_h = nullptr;   // stage 1 stream
-    hzk_ctx* ctx_m = nullptr;   // stage 2 stream
+    std::vector<std::unique_ptr<Pipeline>> pipes;
     std::vector<hzk_g16*> circuits;
     // finished requests' buffers, kept per circuit for reuse (hipFree would
     // synchronise the device and stall the other stage)
@@ -77,8 +86,8 @@ struct hzk_server {
     std::deque<Request*> pending;
     std::map<uint64_t, std::unique_ptr<Request>> all;
     uint64_t next_id = 1;
-    std::thread worker;
     bool running = false, stop = false;
+    long inflight = 0;
     std::string last_error;
     // statistics
     long steps = 0, completed = 0, failed = 0, max_inflight = 0, batched_sum = 0, msm_stages = 0;
@@ -159,6 +168,7 @@ void finish(hzk_server* s, std::vector<Request*>& rs, int status, const hzk_ctx*
         r->status = status;
         r->t_done = t;
         s->in_use -= r->footprint;
+        --s->inflight;
         if (status == 1) ++s->completed; else ++s->failed;
     }
     if (status == 1) {
@@ -166,32 +176,33 @@ void finish(hzk_server* s, std::vector<Request*>& rs, int status, const hzk_ctx*
         ++s->msm_stages;
     }
     s->cv_done.notify_all();
+    s->cv_work.notify_all();   // memory freed: waiting pipelines may admit
 }
 
 // stage 1: upload z and enqueue SpMV + H pipeline for the batch (ctx_h stream)
-int stage_h(hzk_server* s, std::vector<Request*>& rs) {
+int stage_h(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     if (rs.empty()) return 0;
     std::vector<G16Req> reqs;
     for (Request* r : rs) {
-        HZK_HIP(s->ctx_h, hipMemcpyAsync(r->dz, r->z.data(), r->z.size(), hipMemcpyHostToDevice, s->ctx_h->stream));
+        HZK_HIP(ctx, hipMemcpyAsync(r->dz, r->z.data(), r->z.size(), hipMemcpyHostToDevice, ctx->stream));
         reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
     }
-    return g16_h_batch(s->ctx_h, reqs.data(), reqs.size());
+    return g16_h_batch(ctx, reqs.data(), reqs.size());
 }
 
 // stage 2: batched MSMs and host assembly (ctx_m stream; returns when done)
-int stage_msm(hzk_server* s, std::vector<Request*>& rs) {
+int stage_msm(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     if (rs.empty()) return 0;
     std::vector<G16Req> reqs;
     for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
-    int rc = g16_msm_batch(s->ctx_m, reqs.data(), reqs.size(), 0);
+    int rc = g16_msm_batch(ctx, reqs.data(), reqs.size(), 0);
     if (rc) return rc;
 #pragma omp parallel for schedule(dynamic, 1)
     for (long i = 0; i < (long)rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, rs[i]->proof);
     return 0;
 }
 
-void worker_loop(hzk_server* s) {
+void worker_loop(hzk_server* s, Pipeline* p) {
     (void)hipSetDevice(s->device);
     std::vector<Request*> stage1, stage2;
     int mode = 0;
@@ -199,12 +210,13 @@ void worker_loop(hzk_server* s) {
         std::vector<Request*> admitted;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !stage1.empty(); });
-            if (s->stop && s->pending.empty() && stage1.empty()) break;
+            // in sequential mode only the first pipeline proves
+            auto may_take = [&] { return !s->pending.empty() && (s->mode == 0 || p->index == 0); };
+            s->cv_work.wait(lk, [&] { return s->stop || may_take() || !stage1.empty(); });
+            if (s->stop && !may_take() && stage1.empty()) break;
             mode = s->mode;
             size_t cap = mode == 1 ? 1 : (size_t)s->max_batch;
-            size_t inflight = stage1.size();
-            while (!s->pending.empty() && admitted.size() + (mode == 1 ? inflight : 0) < cap) {
+            while (may_take() && admitted.size() < cap) {
                 Request* r = s->pending.front();
                 if (r->footprint > s->budget) {   // can never fit: fail it
                     s->pending.pop_front();
@@ -217,10 +229,15 @@ void worker_loop(hzk_server* s) {
                 if (s->in_use + r->footprint > s->budget) break;   // wait for memory
                 s->pending.pop_front();
                 s->in_use += r->footprint;
+                ++s->inflight;
                 admitted.push_back(r);
             }
-            long inf = (long)(admitted.size() + stage1.size());
-            if (inf > s->max_inflight) s->max_inflight = inf;
+            if (s->inflight > s->max_inflight) s->max_inflight = s->inflight;
+            if (admitted.empty() && stage1.empty()) {   // memory held by another pipeline: wait for a finish
+                s->cv_work.wait(lk);
+                continue;
+            }
+            ++s->steps;
         }
         // device buffers for the admitted requests
         std::vector<Request*> ok;
@@ -232,18 +249,14 @@ void worker_loop(hzk_server* s) {
                 ok.push_back(r);
             }
         }
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            ++s->steps;
-        }
         if (mode == 1) {
             // plain: one request through both stages, nothing overlapped
-            const hzk_ctx* ec = s->ctx_h;
-            int rc = stage_h(s, ok);
-            if (!rc) rc = hipStreamSynchronize(s->ctx_h->stream) == hipSuccess ? 0 : HZK_ERR_HIP;
+            const hzk_ctx* ec = p->ctx_h;
+            int rc = stage_h(s, p->ctx_h, ok);
+            if (!rc) rc = hipStreamSynchronize(p->ctx_h->stream) == hipSuccess ? 0 : HZK_ERR_HIP;
             if (!rc) {
-                ec = s->ctx_m;
-                rc = stage_msm(s, ok);
+                ec = p->ctx_m;
+                rc = stage_msm(s, p->ctx_m, ok);
             }
             for (Request* r : ok) free_request(s, r);
             finish(s, ok, rc ? rc : 1, ec);
@@ -252,18 +265,39 @@ void worker_loop(hzk_server* s) {
         stage2.swap(stage1);
         stage1.swap(ok);
         // stage 1 enqueued first so its kernels run while stage 2's MSMs do
-        int rc1 = stage_h(s, stage1);
-        int rc2 = stage_msm(s, stage2);
-        if (hipStreamSynchronize(s->ctx_h->stream) != hipSuccess && !rc1) rc1 = HZK_ERR_HIP;
+        int rc1 = stage_h(s, p->ctx_h, stage1);
+        int rc2 = stage_msm(s, p->ctx_m, stage2);
+        if (hipStreamSynchronize(p->ctx_h->stream) != hipSuccess && !rc1) rc1 = HZK_ERR_HIP;
         for (Request* r : stage2) free_request(s, r);
-        finish(s, stage2, rc2 ? rc2 : 1, s->ctx_m);
+        finish(s, stage2, rc2 ? rc2 : 1, p->ctx_m);
         stage2.clear();
         if (rc1) {
             for (Request* r : stage1) free_request(s, r);
-            finish(s, stage1, rc1, s->ctx_h);
+            finish(s, stage1, rc1, p->ctx_h);
             stage1.clear();
         }
     }
+}
+
+void destroy_pipes(hzk_server* s) {
+    for (auto& p : s->pipes) {
+        hzk_ctx_destroy(p->ctx_h);
+        hzk_ctx_destroy(p->ctx_m);
+        p->ctx_h = p->ctx_m = nullptr;
+    }
+}
+
+int add_pipe(hzk_server* s) {
+    auto p = std::make_unique<Pipeline>();
+    p->index = (int)s->pipes.size();
+    int rc = hzk_ctx_create(s->device, nullptr, &p->ctx_h);
+    if (!rc) rc = hzk_ctx_create(s->device, nullptr, &p->ctx_m);
+    if (rc) {
+        hzk_ctx_destroy(p->ctx_h);
+        return rc;
+    }
+    s->pipes.push_back(std::move(p));
+    return 0;
 }
 
 }  // namespace
@@ -277,10 +311,10 @@ int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hz
     s->device = device;
     s->mode = mode;
     s->max_batch = max_batch;
-    int rc = hzk_ctx_create(device, nullptr, &s->ctx_h);
-    if (!rc) rc = hzk_ctx_create(device, nullptr, &s->ctx_m);
+    int rc = add_pipe(s);
+    if (!rc) rc = add_pipe(s);   // two pipelines by default
     if (rc) {
-        hzk_ctx_destroy(s->ctx_h);
+        destroy_pipes(s);
         delete s;
         return rc;
     }
@@ -291,14 +325,29 @@ int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hz
     return HZK_OK;
 }
 
+int hzk_server_set_pipelines(hzk_server* s, int n) {
+    if (!s || n < 1 || n > 8 || s->running) return HZK_ERR_INVALID;
+    while ((int)s->pipes.size() > n) {
+        hzk_ctx_destroy(s->pipes.back()->ctx_h);
+        hzk_ctx_destroy(s->pipes.back()->ctx_m);
+        s->pipes.pop_back();
+    }
+    while ((int)s->pipes.size() < n) {
+        int rc = add_pipe(s);
+        if (rc) return rc;
+    }
+    return HZK_OK;
+}
+
 const char* hzk_server_last_error(const hzk_server* s) { return s ? s->last_error.c_str() : "null server"; }
 
 int hzk_server_add_circuit(hzk_server* s, uint64_t seed, int log_m, int num_public, int* circuit_id) {
     if (!s || !circuit_id || s->running) return HZK_ERR_INVALID;
     hzk_g16* g = nullptr;
-    int rc = hzk_g16_setup(s->ctx_m, seed, log_m, num_public, &g);
+    hzk_ctx* ctx = s->pipes[0]->ctx_m;
+    int rc = hzk_g16_setup(ctx, seed, log_m, num_public, &g);
     if (rc) {
-        s->last_error = hzk_last_error(s->ctx_m);
+        s->last_error = hzk_last_error(ctx);
         return rc;
     }
     s->circuits.push_back(g);
@@ -327,13 +376,14 @@ int hzk_server_set_mode(hzk_server* s, int mode) {
         return HZK_ERR_INVALID;
     }
     s->mode = mode;
+    s->cv_work.notify_all();
     return HZK_OK;
 }
 
 int hzk_server_start(hzk_server* s) {
     if (!s || s->running) return HZK_ERR_INVALID;
     s->running = true;
-    s->worker = std::thread(worker_loop, s);
+    for (auto& p : s->pipes) p->th = std::thread(worker_loop, s, p.get());
     return HZK_OK;
 }
 
@@ -352,7 +402,7 @@ int hzk_server_submit(hzk_server* s, int circuit_id, const void* z_host, const v
     *request_id = r->id;
     s->pending.push_back(r.get());
     s->all[r->id] = std::move(r);
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return HZK_OK;
 }
 
@@ -372,7 +422,7 @@ int hzk_server_wait(hzk_server* s, uint64_t request_id, double timeout_s, void* 
     return st;
 }
 
-// {steps, completed, failed, max in flight, mean requests per MSM stage, budget bytes, bytes in use}
+// {steps, completed, failed, max in flight, mean requests per MSM stage, budget bytes, bytes admitted}
 int hzk_server_stats(hzk_server* s, double* out7) {
     if (!s || !out7) return HZK_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -388,7 +438,7 @@ int hzk_server_stats(hzk_server* s, double* out7) {
 
 int hzk_server_inject_failure(hzk_server* s, long after_n) {
     if (!s) return HZK_ERR_INVALID;
-    return hzk_inject_alloc_failure(s->ctx_m, after_n);
+    return hzk_inject_alloc_failure(s->pipes[0]->ctx_m, after_n);
 }
 
 void hzk_server_destroy(hzk_server* s) {
@@ -398,13 +448,13 @@ void hzk_server_destroy(hzk_server* s) {
         s->stop = true;
         s->cv_work.notify_all();
     }
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& p : s->pipes)
+        if (p->th.joinable()) p->th.join();
     for (auto& kv : s->all) free_request(s, kv.second.get());
     for (auto& fl : s->free_bufs)
         for (BufSet& b : fl) release(b);
     for (hzk_g16* g : s->circuits) hzk_g16_free(g);
-    hzk_ctx_destroy(s->ctx_h);
-    hzk_ctx_destroy(s->ctx_m);
+    destroy_pipes(s);
     delete s;
 }
 
