@@ -37,30 +37,34 @@ struct xyzz_t {
     }
     HZK_DEV xyzz_t neg() const { return {X, Y.neg(), ZZ, ZZZ}; }
 
+    // Products are issued in groups of independent ones (F::mul2 / mul3 /
+    // sqr2), which the field layer runs as interleaved chains.
+
     // 2 * (x, y) for a finite affine point (mdbl-2008-s-1, a = 0)
     static HZK_DEV xyzz_t dbl_affine(const affine_t<F>& a) {
-        F U = a.y.dbl();
-        F V = U.sqr();
-        F W = U * V;
-        F S = a.x * V;
-        F xx = a.x.sqr();
+        F U = a.y.dbl(), V, xx;
+        F::sqr2(V, U, xx, a.x);
         F M = xx.dbl() + xx;
-        F X3 = M.sqr() - S.dbl();
-        F Y3 = M * (S - X3) - W * a.y;
-        return {X3, Y3, V, W};
+        F W, S, MM;
+        F::mul3(W, U, V, S, a.x, V, MM, M, M);
+        F X3 = MM - S.dbl();
+        F t, Wy;
+        F::mul2(t, M, S - X3, Wy, W, a.y);
+        return {X3, t - Wy, V, W};
     }
 
     // doubling (dbl-2008-s-1, a = 0); infinity stays infinity (ZZ = 0 propagates)
     HZK_DEV xyzz_t dbl() const {
-        F U = Y.dbl();
-        F V = U.sqr();
-        F W = U * V;
-        F S = X * V;
-        F xx = X.sqr();
+        F U = Y.dbl(), V, xx;
+        F::sqr2(V, U, xx, X);
+        F W, S, ZZ3;
+        F::mul3(W, U, V, S, X, V, ZZ3, V, ZZ);
         F M = xx.dbl() + xx;
-        F X3 = M.sqr() - S.dbl();
-        F Y3 = M * (S - X3) - W * Y;
-        return {X3, Y3, V * ZZ, W * ZZZ};
+        F MM, ZZZ3, Wy;
+        F::mul3(MM, M, M, ZZZ3, W, ZZZ, Wy, W, Y);
+        F X3 = MM - S.dbl();
+        F Y3 = M * (S - X3) - Wy;
+        return {X3, Y3, ZZ3, ZZZ3};
     }
 
     // this += affine (madd-2008-s) with every special case handled:
@@ -68,8 +72,8 @@ struct xyzz_t {
     HZK_DEV void add_affine(const affine_t<F>& a) {
         if (a.is_inf()) return;
         if (is_inf()) { *this = from_affine(a); return; }
-        F U2 = a.x * ZZ;
-        F S2 = a.y * ZZZ;
+        F U2, S2;
+        F::mul2(U2, a.x, ZZ, S2, a.y, ZZZ);
         F P = U2 - X;
         F R = S2 - Y;
         if (P.is_zero()) {
@@ -77,25 +81,27 @@ struct xyzz_t {
             else *this = inf();
             return;
         }
-        F PP = P.sqr();
-        F PPP = P * PP;
-        F Q = X * PP;
-        F X3 = R.sqr() - PPP - Q.dbl();
-        F Y3 = R * (Q - X3) - Y * PPP;
-        ZZ = ZZ * PP;
-        ZZZ = ZZZ * PPP;
+        F PP, RR;
+        F::sqr2(PP, P, RR, R);
+        F PPP, Q, ZZ3;
+        F::mul3(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);
+        F X3 = RR - PPP - Q.dbl();
+        F t, YP, ZZZ3;
+        F::mul3(t, R, Q - X3, YP, Y, PPP, ZZZ3, ZZZ, PPP);
         X = X3;
-        Y = Y3;
+        Y = t - YP;
+        ZZ = ZZ3;
+        ZZZ = ZZZ3;
     }
 
     // this += o (add-2008-s) with every special case handled
     HZK_DEV void add(const xyzz_t& o) {
         if (o.is_inf()) return;
         if (is_inf()) { *this = o; return; }
-        F U1 = X * o.ZZ;
-        F U2 = o.X * ZZ;
-        F S1 = Y * o.ZZZ;
-        F S2 = o.Y * ZZZ;
+        F U1, U2, S1;
+        F::mul3(U1, X, o.ZZ, U2, o.X, ZZ, S1, Y, o.ZZZ);
+        F S2, ZZ12, ZZZ12;
+        F::mul3(S2, o.Y, ZZZ, ZZ12, ZZ, o.ZZ, ZZZ12, ZZZ, o.ZZZ);
         F P = U2 - U1;
         F R = S2 - S1;
         if (P.is_zero()) {
@@ -103,15 +109,17 @@ struct xyzz_t {
             else *this = inf();
             return;
         }
-        F PP = P.sqr();
-        F PPP = P * PP;
-        F Q = U1 * PP;
-        F X3 = R.sqr() - PPP - Q.dbl();
-        F Y3 = R * (Q - X3) - S1 * PPP;
-        ZZ = ZZ * o.ZZ * PP;
-        ZZZ = ZZZ * o.ZZZ * PPP;
+        F PP, RR;
+        F::sqr2(PP, P, RR, R);
+        F PPP, Q, ZZ3;
+        F::mul3(PPP, P, PP, Q, U1, PP, ZZ3, ZZ12, PP);
+        F X3 = RR - PPP - Q.dbl();
+        F t, SP, ZZZ3;
+        F::mul3(t, R, Q - X3, SP, S1, PPP, ZZZ3, ZZZ12, PPP);
         X = X3;
-        Y = Y3;
+        Y = t - SP;
+        ZZ = ZZ3;
+        ZZZ = ZZZ3;
     }
 };
 

@@ -40,6 +40,59 @@ struct FrParams {
                                        0x53bb8085, 0x8c49833d, 0x7f4e44a5, 0x0216d0b1};
 };
 
+// Carry-folding multiply-accumulate steps of 2 or 3 independent Montgomery
+// products, interleaved: every chain's v_mad_u64_u32 first, then every
+// chain's v_addc_co_u32. Each chain carries through its own SGPR pair, so the
+// two wait states gfx950 needs between a VALU SGPR write and its use as
+// carry-in are filled by the other chains' multiplies (one s_nop covers the
+// two-chain case), and the compiler sees independent dependency chains
+// instead of one chain serialised through VCC.
+HZK_DEV void mac2_vv(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %8, %9, %1\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %2, %4, 0, %2, %4\n\t"
+        "v_addc_co_u32_e64 %3, %5, 0, %3, %5"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(top[0]), "+v"(top[1]), "=&s"(c0), "=&s"(c1)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
+}
+HZK_DEV void mac2_vs(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t x1, uint32_t ys) {
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %8, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %7, %8, %1\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %2, %4, 0, %2, %4\n\t"
+        "v_addc_co_u32_e64 %3, %5, 0, %3, %5"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(top[0]), "+v"(top[1]), "=&s"(c0), "=&s"(c1)
+        : "v"(x0), "v"(x1), "s"(ys));
+}
+HZK_DEV void mac3_vv(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t x2,
+                     uint32_t y2) {
+    uint64_t c0, c1, c2;
+    asm("v_mad_u64_u32 %0, %6, %9, %10, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %11, %12, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %13, %14, %2\n\t"
+        "v_addc_co_u32_e64 %3, %6, 0, %3, %6\n\t"
+        "v_addc_co_u32_e64 %4, %7, 0, %4, %7\n\t"
+        "v_addc_co_u32_e64 %5, %8, 0, %5, %8"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(top[0]), "+v"(top[1]), "+v"(top[2]), "=&s"(c0), "=&s"(c1),
+          "=&s"(c2)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2));
+}
+HZK_DEV void mac3_vs(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t ys) {
+    uint64_t c0, c1, c2;
+    asm("v_mad_u64_u32 %0, %6, %9, %12, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %10, %12, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %11, %12, %2\n\t"
+        "v_addc_co_u32_e64 %3, %6, 0, %3, %6\n\t"
+        "v_addc_co_u32_e64 %4, %7, 0, %4, %7\n\t"
+        "v_addc_co_u32_e64 %5, %8, 0, %5, %8"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(top[0]), "+v"(top[1]), "+v"(top[2]), "=&s"(c0), "=&s"(c1),
+          "=&s"(c2)
+        : "v"(x0), "v"(x1), "v"(x2), "s"(ys));
+}
+
 template <class P>
 struct fp {
     uint32_t l[8];
@@ -156,11 +209,82 @@ struct fp {
         for (int i = 0; i < 8; ++i) r[i] = t[i];
     }
 
+    // N = 2 or 3 independent Montgomery products in one interleaved scan
+    // (same FIPS schedule as mont_mul, chain c computing r[c] = a[c] b[c] / R)
+    template <int N>
+    static HZK_DEV void mont_mul_n(uint32_t (*r)[8], const uint32_t* const* a, const uint32_t* const* b) {
+        static_assert(N == 2 || N == 3, "2 or 3 chains");
+        uint32_t m[N][8], t[N][8];
+        uint64_t acc[N];
+        uint32_t top[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) { acc[c] = 0; top[c] = 0; }
+        auto vv = [&](int i, int j) {
+            if constexpr (N == 2) mac2_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j]);
+            else mac3_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j], a[2][i], b[2][j]);
+        };
+        auto vs = [&](int j, uint32_t ys) {
+            if constexpr (N == 2) mac2_vs(acc, top, m[0][j], m[1][j], ys);
+            else mac3_vs(acc, top, m[0][j], m[1][j], m[2][j], ys);
+        };
+        auto shift = [&]() {
+#pragma unroll
+            for (int c = 0; c < N; ++c) { acc[c] = (acc[c] >> 32) | ((uint64_t)top[c] << 32); top[c] = 0; }
+        };
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+#pragma unroll
+            for (int j = 0; j < k; ++j) vs(j, P::MOD[k - j]);
+#pragma unroll
+            for (int j = 0; j <= k; ++j) vv(j, k - j);
+#pragma unroll
+            for (int c = 0; c < N; ++c) m[c][k] = (uint32_t)acc[c] * P::M0;
+            vs(k, P::MOD[0]);
+            shift();
+        }
+#pragma unroll
+        for (int k = 8; k < 15; ++k) {
+#pragma unroll
+            for (int j = k - 7; j < 8; ++j) vv(j, k - j);
+#pragma unroll
+            for (int j = k - 7; j < 8; ++j) vs(j, P::MOD[k - j]);
+#pragma unroll
+            for (int c = 0; c < N; ++c) t[c][k - 8] = (uint32_t)acc[c];
+            shift();
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            t[c][7] = (uint32_t)acc[c];
+            final_sub(t[c]);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) r[c][i] = t[c][i];
+        }
+    }
+    // r0 = a0 b0, r1 = a1 b1 (and r2 = a2 b2) as interleaved chains
+    static HZK_DEV void mul2(fp& r0, const fp& a0, const fp& b0, fp& r1, const fp& a1, const fp& b1) {
+        const uint32_t* a[2] = {a0.l, a1.l};
+        const uint32_t* b[2] = {b0.l, b1.l};
+        uint32_t r[2][8];
+        mont_mul_n<2>(r, a, b);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { r0.l[i] = r[0][i]; r1.l[i] = r[1][i]; }
+    }
+    static HZK_DEV void mul3(fp& r0, const fp& a0, const fp& b0, fp& r1, const fp& a1, const fp& b1, fp& r2, const fp& a2,
+                             const fp& b2) {
+        const uint32_t* a[3] = {a0.l, a1.l, a2.l};
+        const uint32_t* b[3] = {b0.l, b1.l, b2.l};
+        uint32_t r[3][8];
+        mont_mul_n<3>(r, a, b);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { r0.l[i] = r[0][i]; r1.l[i] = r[1][i]; r2.l[i] = r[2][i]; }
+    }
+
     friend HZK_DEV fp operator*(const fp& a, const fp& b) { fp r; mont_mul(r.l, a.l, b.l); return r; }
     HZK_DEV fp& operator*=(const fp& b) { mont_mul(l, l, b.l); return *this; }
     HZK_DEV fp& operator+=(const fp& b) { *this = *this + b; return *this; }
     HZK_DEV fp& operator-=(const fp& b) { *this = *this - b; return *this; }
     HZK_DEV fp sqr() const { fp r; mont_mul(r.l, l, l); return r; }
+    static HZK_DEV void sqr2(fp& r0, const fp& a0, fp& r1, const fp& a1) { mul2(r0, a0, a0, r1, a1, a1); }
 
     // Montgomery form <-> canonical integer
     HZK_DEV fp from_mont() const {   // a * 1 * R^-1
@@ -191,13 +315,10 @@ struct fp {
 typedef fp<FqParams> fq_t;
 typedef fp<FrParams> fr_t;
 
-// Out-of-line Fq product used by Fq2: a G2 point operation inlines ~30 Fq
-// products, which would overflow the instruction cache; one shared body keeps
-// the G2 loops resident at the cost of a call per product.
-static __device__ __noinline__ fq_t fq_mul_call(fq_t a, fq_t b) { return a * b; }
-
-// Fq2 = Fq[u]/(u^2 + 1), one element per lane (no cross-lane traffic).
-// Fq2 = Fq[u]/(u^2 + 1), one element per lane (no cross-lane traffic).
+// Fq2 = Fq[u]/(u^2 + 1), one element per lane (no cross-lane traffic). A G2
+// point operation would inline ~30 Fq products and overflow the instruction
+// cache, so the Fq2 product and square are out-of-line calls, each running its
+// Fq products as interleaved chains.
 struct fq2_t {
     fq_t c0, c1;
     static HZK_DEV fq2_t zero() { return {fq_t::zero(), fq_t::zero()}; }
@@ -209,20 +330,37 @@ struct fq2_t {
     HZK_DEV fq2_t dbl() const { return {c0.dbl(), c1.dbl()}; }
     HZK_DEV fq2_t neg() const { return {c0.neg(), c1.neg()}; }
     HZK_DEV fq2_t cneg(bool f) const { return f ? neg() : *this; }
-    // Karatsuba: 3 Fq multiplications
-    friend HZK_DEV fq2_t operator*(const fq2_t& a, const fq2_t& b) {
-        fq_t aa = fq_mul_call(a.c0, b.c0), bb = fq_mul_call(a.c1, b.c1);
-        fq_t s = fq_mul_call(a.c0 + a.c1, b.c0 + b.c1);
+    // Karatsuba: 3 Fq multiplications, interleaved in one call
+    static __device__ __noinline__ fq2_t mul_call(fq2_t a, fq2_t b) {
+        fq_t aa, bb, s;
+        fq_t::mul3(aa, a.c0, b.c0, bb, a.c1, b.c1, s, a.c0 + a.c1, b.c0 + b.c1);
         return {aa - bb, s - aa - bb};
+    }
+    // complex squaring: 2 Fq multiplications, interleaved in one call
+    static __device__ __noinline__ fq2_t sqr_call(fq2_t a) {
+        fq_t ab, t;
+        fq_t::mul2(ab, a.c0, a.c1, t, a.c0 + a.c1, a.c0 - a.c1);
+        return {t, ab.dbl()};
+    }
+    friend HZK_DEV fq2_t operator*(const fq2_t& a, const fq2_t& b) { return mul_call(a, b); }
+    static HZK_DEV void mul2(fq2_t& r0, const fq2_t& a0, const fq2_t& b0, fq2_t& r1, const fq2_t& a1, const fq2_t& b1) {
+        r0 = mul_call(a0, b0);
+        r1 = mul_call(a1, b1);
+    }
+    static HZK_DEV void mul3(fq2_t& r0, const fq2_t& a0, const fq2_t& b0, fq2_t& r1, const fq2_t& a1, const fq2_t& b1,
+                             fq2_t& r2, const fq2_t& a2, const fq2_t& b2) {
+        r0 = mul_call(a0, b0);
+        r1 = mul_call(a1, b1);
+        r2 = mul_call(a2, b2);
+    }
+    static HZK_DEV void sqr2(fq2_t& r0, const fq2_t& a0, fq2_t& r1, const fq2_t& a1) {
+        r0 = sqr_call(a0);
+        r1 = sqr_call(a1);
     }
     HZK_DEV fq2_t& operator*=(const fq2_t& b) { *this = *this * b; return *this; }
     HZK_DEV fq2_t& operator+=(const fq2_t& b) { *this = *this + b; return *this; }
     HZK_DEV fq2_t& operator-=(const fq2_t& b) { *this = *this - b; return *this; }
-    // complex squaring: 2 Fq multiplications
-    HZK_DEV fq2_t sqr() const {
-        fq_t ab = fq_mul_call(c0, c1);
-        return {fq_mul_call(c0 + c1, c0 - c1), ab.dbl()};
-    }
+    HZK_DEV fq2_t sqr() const { return sqr_call(*this); }
     HZK_DEV fq2_t inverse() const {
         fq_t t = (c0.sqr() + c1.sqr()).inverse();
         return {c0 * t, (c1 * t).neg()};

@@ -44,8 +44,10 @@ HZK_DEV uint32_t find_key(const uint32_t* __restrict__ offs, uint32_t nk, uint32
 // to bucket[k]; the partial of a bucket that began in an earlier chunk goes
 // to carry[2c] (head), that of a bucket continuing into the next chunk to
 // carry[2c+1] (tail). Persistent grid over the whole batch's chunk space.
+// at least two waves per SIMD: the G2 lane state does not fit twice in 512
+// registers otherwise (occupancy 1 left the G2 accumulation latency-bound)
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uint32_t* __restrict__ sorted,
+__global__ void __launch_bounds__(256, 2) k_msm_accumulate(MsmBatchDev b, const uint32_t* __restrict__ sorted,
                                                         const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ bucket,
                                                         xyzz_t<F>* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
@@ -75,14 +77,16 @@ __global__ void __launch_bounds__(256) k_msm_accumulate(MsmBatchDev b, const uin
         for (uint32_t e = e0; e < e1; ++e) {
             bool neg = idx >> 31;
             affine_t<F> Q = P;
+            constexpr bool prefetch = sizeof(F) == 32;   // G2: registers are scarcer than latency
             if (e + 1 < e1) {   // prefetch the next base while this one is added
                 const uint32_t t = (e + 1) & 3;
                 if (t == 0) q = srt4[(e + 1) >> 2];
                 idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
-                P = bases[idx & 0x7fffffffu];
+                if (prefetch) P = bases[idx & 0x7fffffffu];
             }
             if (neg) Q.y = Q.y.neg();
             acc.add_affine(Q);
+            if (!prefetch && e + 1 < e1) P = bases[idx & 0x7fffffffu];
             if (e + 1 == next || e + 1 == e1) {
                 if (before) carry[2 * gid] = acc;
                 else if (next > e1) carry[2 * gid + 1] = acc;
@@ -132,7 +136,7 @@ __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint3
 
 // lane per bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l)
 template <class F>
-__global__ void __launch_bounds__(256) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
+__global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
                                                    const xyzz_t<F>* __restrict__ carry, xyzz_t<F>* __restrict__ bucket) {
     const uint64_t* kbeg = space_g(b, SP_KEY);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
