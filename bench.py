@@ -48,7 +48,7 @@ def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
     """Oracle naive MSM on a bounded sample of the same inputs (the only oracle use here)."""
     from oracle import orc
     nthreads = orc.lib().orc_num_threads()
-    n0 = 1 << 10
+    n0 = 1 << 14
     t = time.perf_counter()
     orc.msm_g1(scalars_host[: 4 * n0].copy(), bases_dev.download(np.uint64, 8 * n0), 0)
     per_pt = (time.perf_counter() - t) / n0

@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint3
     const uint64_t* beg = space(b, SP_TILEWG);
     uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (d.NB >= MSM_TWO_PASS_NB) return;   // scattered in two passes below
+    if (msm_two_pass(d.n, d.NB)) return;   // scattered in two passes below
     uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     uint32_t t = local % d.T, j = local / d.T;
     const uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_coarse(MsmBatchDev b
     const uint64_t* beg = space(b, SP_TILEWG);
     const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (d.NB < MSM_TWO_PASS_NB) return;
+    if (!msm_two_pass(d.n, d.NB)) return;
     const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     const uint32_t t = local % d.T, j = local / d.T;
     const uint32_t FB = d.NB / MSM_COARSE, fshift = __ffs(FB) - 1;
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_fine(MsmBatchDev b, 
     const uint64_t* beg = space(b, SP_CGROUP);
     const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (d.NB < MSM_TWO_PASS_NB) return;
+    if (!msm_two_pass(d.n, d.NB)) return;
     const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     const uint32_t FB = d.NB / MSM_COARSE;
     const uint32_t* of = offs + space(b, SP_KEY)[ii] + (size_t)local * FB;   // local = j * 256 + h
@@ -485,8 +485,10 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     if (force_g) g = force_g;
     p.g = g;
     // scalar tiles per window for the LDS histogram / scatter workgroups
+    // (about 512 workgroups per MSM; every tile at least 4096 scalars and 4x
+    // the bucket count, so the per-tile histograms stay smaller than the digits)
     uint32_t T = (512 + p.W - 1) / p.W;
-    while (T > 1 && n / T < 4096) T >>= 1;
+    while (T > 1 && (n / T < 4096 || n / T < 4 * (size_t)p.NB)) T >>= 1;
     p.T = T;
     p.tile = (uint32_t)((n + T - 1) / T);
     return p;

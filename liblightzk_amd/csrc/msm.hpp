@@ -67,9 +67,11 @@ struct MsmScalarBufs {
     uint32_t* sorted;    // SP_ENTRY
     uint2* coarse;       // SP_ENTRY 8-byte entries, or null when no instance uses the two-pass scatter
 };
-// instances with at least this many buckets per window scatter in two passes
-constexpr uint32_t MSM_TWO_PASS_NB = 2048;
 constexpr uint32_t MSM_COARSE = 256;   // coarse key groups per window
+// Instances with many buckets per window and enough points that each coarse
+// group holds a few thousand entries scatter in two passes; below that the
+// per-group fixed cost outweighs the write coalescing.
+__host__ __device__ inline bool msm_two_pass(uint32_t n, uint32_t NB) { return NB >= 2048 && n >= (1u << 19); }
 void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t* host_totals, uint32_t max_nb,
                               const MsmScalarBufs& bufs);
 constexpr uint32_t MSM_SCAN_TILE = 1024;

@@ -3,6 +3,7 @@
 // translation unit. Stages 1-4 and the batch layout are described in msm.hip.
 #pragma once
 #include <omp.h>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -109,29 +110,36 @@ template <class F>
 __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint32_t* __restrict__ offs,
                                                       const uint32_t* __restrict__ chunk2key, uint32_t stride,
                                                       xyzz_t<F>* __restrict__ carry) {
+    // grid-stride over the chunk space: a small grid, since most instances
+    // have no hot bucket and their lanes leave at once
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
-    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= cbeg[b.ninst]) return;
-    uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
-    const MsmInst& d = b.inst[ii];
-    if (b.stats[4 * ii + 2] <= HOT * d.K) return;   // no bucket of this instance is hot
-    const uint32_t c = (uint32_t)(gid - cbeg[ii]);
-    if ((uint64_t)c * d.K >= b.stats[4 * ii]) return;
-    const uint32_t* of = offs + space_g(b, SP_KEY)[ii];
-    const uint32_t k = chunk2key[gid], start = of[k];
-    if (start >= c * d.K) return;   // chunk c holds no head partial
-    const uint32_t f = start / d.K, l = (of[k + 1] - 1) / d.K;
-    if (l - f <= HOT) return;
-    const uint32_t rel = c - (f + 1), span = l - f;
-    if (rel % (stride * K2) != 0) return;
-    xyzz_t<F>* base = carry + 2 * cbeg[ii];
-    xyzz_t<F> acc = base[2 * c];
-    for (int i = 1; i < K2; ++i) {
-        uint32_t r = rel + i * stride;
-        if (r >= span) break;
-        acc.add(base[2 * (c + i * stride)]);
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < cbeg[b.ninst];
+         gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
+        const MsmInst& d = b.inst[ii];
+        if (b.stats[4 * ii + 2] <= HOT * d.K) {   // no bucket of this instance is hot: skip its chunks
+            const uint64_t S = (uint64_t)gridDim.x * blockDim.x, rem = cbeg[ii + 1] - gid;
+            gid += ((rem + S - 1) / S - 1) * S;   // the loop step then lands on the first chunk past ii
+            continue;
+        }
+        const uint32_t c = (uint32_t)(gid - cbeg[ii]);
+        if ((uint64_t)c * d.K >= b.stats[4 * ii]) continue;
+        const uint32_t* of = offs + space_g(b, SP_KEY)[ii];
+        const uint32_t k = chunk2key[gid], start = of[k];
+        if (start >= c * d.K) continue;   // chunk c holds no head partial
+        const uint32_t f = start / d.K, l = (of[k + 1] - 1) / d.K;
+        if (l - f <= HOT) continue;
+        const uint32_t rel = c - (f + 1), span = l - f;
+        if (rel % (stride * K2) != 0) continue;
+        xyzz_t<F>* base = carry + 2 * cbeg[ii];
+        xyzz_t<F> acc = base[2 * c];
+        for (int i = 1; i < K2; ++i) {
+            uint32_t r = rel + i * stride;
+            if (r >= span) break;
+            acc.add(base[2 * (c + i * stride)]);
+        }
+        base[2 * c] = acc;
     }
-    base[2 * c] = acc;
 }
 
 // lane per bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l)
@@ -264,8 +272,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_WIN] = d.W;
         sz[SP_ENTRY] = (entries + 63) & ~(uint64_t)63;   // 256-B aligned per instance (vector index loads)
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
-        sz[SP_CGROUP] = d.NB >= MSM_TWO_PASS_NB ? (uint64_t)d.W * MSM_COARSE : 0;
-        if (d.NB >= MSM_TWO_PASS_NB) L.two_pass = true;
+        sz[SP_CGROUP] = msm_two_pass(d.n, d.NB) ? (uint64_t)d.W * MSM_COARSE : 0;
+        if (msm_two_pass(d.n, d.NB)) L.two_pass = true;
         for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
         if (d.NB > L.max_nb) L.max_nb = d.NB;
         uint64_t maxspan = (d.n + d.K - 1) / d.K + 1;   // a bucket holds at most n entries
@@ -332,7 +340,7 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
         TimedLaunch tl(ctx, "msm_fixup");
         uint32_t stride = 1;
         for (uint32_t lv = 0; lv < L.tree_levels; ++lv, stride *= K2)
-            hipLaunchKernelGGL(k_msm_hot_tree<F>, dim3(blocks_for(L.tot[SP_CHUNK], 256)), dim3(256), 0, st, b,
+            hipLaunchKernelGGL(k_msm_hot_tree<F>, dim3(std::min<unsigned>(blocks_for(L.tot[SP_CHUNK], 256), 1024)), dim3(256), 0, st, b,
                                (const uint32_t*)sb.offs, (const uint32_t*)chunk2key, stride, carry);
         hipLaunchKernelGGL(k_msm_fixup<F>, dim3(blocks_for(L.tot[SP_KEY], 256)), dim3(256), 0, st, b, (const uint32_t*)sb.offs,
                            (const xyzz_t<F>*)carry, bucket);
