@@ -90,6 +90,118 @@ def groth16_plain(ctx, log_m, proofs):
             "kernels_ms_per_proof": {k: v[0] / proofs for k, v in rep.items()}}
 
 
+def batched_forms(ctx, count=128, seed=4, reps=3):
+    """128 G1 MSMs with n = floor(2^(10 + 10U)) and 128 Fr NTTs of 2^k, k
+    uniform in 10..20 (seeded stream 6 of seed 4): each set as one batched,
+    hipGraph-replayed call against the same instances launched one at a
+    time; outputs compared byte for byte between the two."""
+    import liblightzk_amd as z
+    rng = np.random.default_rng([seed, 6])
+    sizes = [int(2 ** (10 + 10 * u)) for u in rng.random(count)]
+    nmax = 1 << 20
+    s = z.gen_field(seed, 0, 2 * nmax, "fr")
+    k = z.gen_field(seed, 1, nmax, "fr")
+    d_s, d_k, d_b = ctx.upload(s), ctx.upload(k), ctx.alloc(64 * nmax)
+    ctx.fixbase_g1(d_k, nmax, d_b)
+    offs = [int(o) for o in rng.integers(0, nmax, size=count)]
+    outs_b = ctx.alloc(96 * count)
+    outs_1 = ctx.alloc(96 * count)
+    jobs = [(d_s.ptr + 32 * o, d_b.ptr, m, outs_b.ptr + 96 * i) for i, (o, m) in enumerate(zip(offs, sizes))]
+    ctx.msm_batch(1, jobs, graph=True)   # capture
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.msm_batch(1, jobs, graph=True)
+    ctx.sync()
+    tb = (time.perf_counter() - t) / reps
+    for i, (o, m) in enumerate(zip(offs, sizes)):   # warm plans / scratch
+        ctx.msm_g1(d_s.ptr + 32 * o, d_b, m, outs_1.offset(96 * i))
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        for i, (o, m) in enumerate(zip(offs, sizes)):
+            ctx.msm_g1(d_s.ptr + 32 * o, d_b, m, outs_1.offset(96 * i))
+    ctx.sync()
+    t1 = (time.perf_counter() - t) / reps
+    msm_same = bool(np.array_equal(outs_b.download(np.uint64, 12 * count), outs_1.download(np.uint64, 12 * count)))
+    pts = sum(sizes)
+    # NTTs
+    logs = [int(v) for v in rng.integers(10, 21, size=count)]
+    tot = sum(1 << lg for lg in logs)
+    x = z.gen_field(seed, 8, tot, "fr")
+    d_a, d_c = ctx.upload(x), ctx.upload(x)
+    base, jobs_n = 0, []
+    for lg in logs:
+        jobs_n.append((lg, base))
+        base += 32 << lg
+    ctx.ntt_batch([(d_a.ptr + o, 1 << lg, 0) for lg, o in jobs_n], graph=True)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.ntt_batch([(d_a.ptr + o, 1 << lg, 0) for lg, o in jobs_n], graph=True)
+    ctx.sync()
+    nb = (time.perf_counter() - t) / reps
+    for lg, o in jobs_n:
+        ctx.ntt(d_c.ptr + o, 1 << lg)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        for lg, o in jobs_n:
+            ctx.ntt(d_c.ptr + o, 1 << lg)
+    ctx.sync()
+    n1 = (time.perf_counter() - t) / reps
+    ntt_same = bool(np.array_equal(d_a.download(np.uint64, 4 * tot), d_c.download(np.uint64, 4 * tot)))
+    bfly = sum((1 << lg) // 2 * lg for lg in logs)
+    for b in (d_s, d_k, d_b, outs_b, outs_1, d_a, d_c):
+        b.free()
+    return {"msm": {"instances": count, "points": pts, "sizes": "floor(2^(10+10U)), seeded",
+                    "batched_graph_ms": tb * 1e3, "one_at_a_time_ms": t1 * 1e3,
+                    "batched_Mpoints_per_s": pts / tb / 1e6, "one_at_a_time_Mpoints_per_s": pts / t1 / 1e6,
+                    "speedup": t1 / tb, "outputs_identical": msm_same},
+            "ntt": {"transforms": count, "elements": tot, "sizes": "2^k, k uniform in 10..20, seeded",
+                    "batched_graph_ms": nb * 1e3, "one_at_a_time_ms": n1 * 1e3,
+                    "batched_Gbutterflies_per_s": bfly / nb / 1e9, "one_at_a_time_Gbutterflies_per_s": bfly / n1 / 1e9,
+                    "speedup": n1 / nb, "outputs_identical": ntt_same}}
+
+
+def large_msm_variants(ctx, reps=2):
+    """G2 MSM at n = 2^22 (bases k_i * G2) and the G1 2^24 MSM on sparse scalars
+    (70% in {0, 1}), the other large single-MSM shapes of the workload set."""
+    import liblightzk_amd as z
+    out = {}
+    n = 1 << 22
+    s, k = z.gen_field(3, 0, n, "fr"), z.gen_field(3, 1, n, "fr")
+    d_s, d_k, d_b, d_o = ctx.upload(s), ctx.upload(k), ctx.alloc(128 * n), ctx.alloc(192)
+    ctx.fixbase_g2(d_k, n, d_b)
+    ctx.msm_g2(d_s, d_b, n, d_o)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.msm_g2(d_s, d_b, n, d_o)
+    ctx.sync()
+    el = (time.perf_counter() - t) / reps
+    out["g2_msm_2^22_uniform"] = {"ms": el * 1e3, "Mpoints_per_s": n / el / 1e6}
+    for b in (d_s, d_k, d_b, d_o):
+        b.free()
+    n = 1 << 24
+    s, k = z.gen_field(3, 0, n, "fr_sparse"), z.gen_field(3, 1, n, "fr")
+    d_s, d_k, d_b, d_o = ctx.upload(s), ctx.upload(k), ctx.alloc(64 * n), ctx.alloc(96)
+    ctx.fixbase_g1(d_k, n, d_b)
+    ctx.msm_g1(d_s, d_b, n, d_o)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.msm_g1(d_s, d_b, n, d_o)
+    ctx.sync()
+    el = (time.perf_counter() - t) / reps
+    st = ctx.msm_last_stats()
+    out["g1_msm_2^24_sparse"] = {"ms": el * 1e3, "Mpoints_per_s": n / el / 1e6, "nonzero_digits": st["entries"],
+                                 "largest_bucket": st["max_bucket"]}
+    for b in (d_s, d_k, d_b, d_o):
+        b.free()
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -195,6 +307,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate-mult", type=float, default=4.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU only "
@@ -277,6 +391,12 @@ def main():
     if args.g16_proofs > 0:
         out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs)
         log(f"groth16 plain: {out['groth16_plain']['proofs_per_s']:.1f} proofs/s")
+    if not args.no_batched:
+        out["batched"] = batched_forms(ctx)
+        log(f"batched: msm x{out['batched']['msm']['speedup']:.2f}, ntt x{out['batched']['ntt']['speedup']:.2f}")
+    if not args.no_large:
+        out["large_msm"] = large_msm_variants(ctx)
+        log(f"large: {json.dumps(out['large_msm'])}")
     if args.serve_requests > 0:
         out["groth16_served"] = groth16_served([int(v) for v in args.serve_logs.split(",")],
                                                [float(v) for v in args.serve_weights.split(",")],
