@@ -35,7 +35,7 @@ enum MsmSpace {
     SP_SCANTILE,    // scan tiles over the key space
     SP_CHUNK,       // accumulation chunks of K sorted entries (upper bound)
     SP_BRED,        // bucket-reduction lanes (window x run)
-    SP_WIN,         // windows
+    SP_WIN,         // window-sum partials (S per window)
     SP_ENTRY,       // n * W digit / sorted-entry slots
     SP_HIST,        // T * nk per-tile histogram counters
     SP_CGROUP,      // W * 256 coarse key groups (two-pass scatter)
@@ -47,6 +47,7 @@ struct MsmInst {
     const void* scalars;   // Fr[n], Montgomery
     const void* bases;     // affine points
     uint32_t n, c, W, NB, nk, K, g, G, T, tile;
+    uint32_t S;            // window-sum workgroups per window
 };
 
 struct MsmBatchDev {

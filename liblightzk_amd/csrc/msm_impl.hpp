@@ -142,29 +142,33 @@ __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint3
     }
 }
 
-// lane per bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l)
+// bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l), by
+// the lane of chunk f+1 (the first chunk that continues it)
 template <class F>
 __global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ chunk2key,
                                                    const xyzz_t<F>* __restrict__ carry, xyzz_t<F>* __restrict__ bucket) {
-    const uint64_t* kbeg = space_g(b, SP_KEY);
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= kbeg[b.ninst]) return;
-    uint32_t ii = find_inst_g(kbeg, b.ninst, gid);
+    if (gid >= cbeg[b.ninst]) return;
+    uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
     const MsmInst& d = b.inst[ii];
-    const uint32_t k = (uint32_t)(gid - kbeg[ii]);
-    if (k >= d.nk) return;
-    const uint32_t o0 = offs[gid], o1 = offs[gid + 1];
-    if (o0 == o1) return;
-    const uint32_t f = o0 / d.K, l = (o1 - 1) / d.K;
-    if (f == l) return;   // written by its chunk
-    const xyzz_t<F>* base = carry + 2 * space_g(b, SP_CHUNK)[ii];
+    const uint32_t c = (uint32_t)(gid - cbeg[ii]);
+    if ((uint64_t)c * d.K >= b.stats[4 * ii]) return;
+    const uint64_t kb = space_g(b, SP_KEY)[ii];
+    const uint32_t k = chunk2key[gid], start = offs[kb + k];
+    if (start >= c * d.K) return;   // the chunk's first bucket starts in it
+    const uint32_t f = start / d.K;
+    if (c != f + 1) return;
+    const uint32_t l = (offs[kb + k + 1] - 1) / d.K;
+    const xyzz_t<F>* base = carry + 2 * cbeg[ii];
     xyzz_t<F> acc = base[2 * f + 1];
     if (l - f > HOT) {
         acc.add(base[2 * (f + 1)]);
     } else {
-        for (uint32_t c = f + 1; c <= l; ++c) acc.add(base[2 * c]);
+        for (uint32_t cc = f + 1; cc <= l; ++cc) acc.add(base[2 * cc]);
     }
-    bucket[gid] = acc;
+    bucket[kb + k] = acc;
 }
 
 // ------------------------------------------------------------------ 7. bucket reduction
@@ -202,7 +206,8 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xy
     segsum[gid] = acc;
 }
 
-// block per window: tree sum of the window's run sums
+// S workgroups per window, each a tree sum of G/S of the window's run sums;
+// the host adds the S partials of a window before the combine
 template <class F>
 __global__ void __launch_bounds__(256) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
                                                     xyzz_t<F>* __restrict__ winsum) {
@@ -210,10 +215,11 @@ __global__ void __launch_bounds__(256) k_window_sum(MsmBatchDev b, const xyzz_t<
     const uint64_t* wbeg = space_g(b, SP_WIN);
     uint32_t ii = find_inst_g(wbeg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    uint32_t j = (uint32_t)(blockIdx.x - wbeg[ii]);
-    const xyzz_t<F>* ss = segsum + space_g(b, SP_BRED)[ii] + (size_t)j * d.G;
+    const uint32_t local = (uint32_t)(blockIdx.x - wbeg[ii]);
+    const uint32_t j = local / d.S, part = local % d.S, Gs = d.G / d.S;
+    const xyzz_t<F>* ss = segsum + space_g(b, SP_BRED)[ii] + (size_t)j * d.G + (size_t)part * Gs;
     xyzz_t<F> acc = xyzz_t<F>::inf();
-    for (uint32_t s = threadIdx.x; s < d.G; s += 256) acc.add(ss[s]);
+    for (uint32_t s = threadIdx.x; s < Gs; s += 256) acc.add(ss[s]);
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -261,6 +267,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         d.G = p.NB / p.g;
         d.T = p.T;
         d.tile = p.tile;
+        d.S = 1;   // window-sum partials: at most ~2 run sums per lane of 256
+        while (d.S < 64 && d.G / d.S > 512) d.S *= 2;
         uint64_t entries = (uint64_t)d.n * d.W;
         uint64_t sz[SP_COUNT];
         sz[SP_DIGIT] = d.n;
@@ -269,7 +277,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_SCANTILE] = ((uint64_t)d.nk + MSM_SCAN_TILE) / MSM_SCAN_TILE;
         sz[SP_CHUNK] = (entries + d.K - 1) / d.K;
         sz[SP_BRED] = (uint64_t)d.W * d.G;
-        sz[SP_WIN] = d.W;
+        sz[SP_WIN] = (uint64_t)d.W * d.S;
         sz[SP_ENTRY] = (entries + 63) & ~(uint64_t)63;   // 256-B aligned per instance (vector index loads)
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
         sz[SP_CGROUP] = msm_two_pass(d.n, d.NB) ? (uint64_t)d.W * MSM_COARSE : 0;
@@ -342,8 +350,8 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
         for (uint32_t lv = 0; lv < L.tree_levels; ++lv, stride *= K2)
             hipLaunchKernelGGL(k_msm_hot_tree<F>, dim3(std::min<unsigned>(blocks_for(L.tot[SP_CHUNK], 256), 1024)), dim3(256), 0, st, b,
                                (const uint32_t*)sb.offs, (const uint32_t*)chunk2key, stride, carry);
-        hipLaunchKernelGGL(k_msm_fixup<F>, dim3(blocks_for(L.tot[SP_KEY], 256)), dim3(256), 0, st, b, (const uint32_t*)sb.offs,
-                           (const xyzz_t<F>*)carry, bucket);
+        hipLaunchKernelGGL(k_msm_fixup<F>, dim3(blocks_for(L.tot[SP_CHUNK], 256)), dim3(256), 0, st, b, (const uint32_t*)sb.offs,
+                           (const uint32_t*)chunk2key, (const xyzz_t<F>*)carry, bucket);
     }
     {
         TimedLaunch tl(ctx, "msm_bucket_reduce");
@@ -437,7 +445,8 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HostPt acc = HostPt::inf();
         for (int j = (int)d.W - 1; j >= 0; --j) {
             for (uint32_t q = 0; q < d.c; ++q) acc = acc.dbl();
-            acc = acc.add(HostPt::load_xyzz(hwin + (wbeg[i] + j) * sizeof(xyzz_t<F>)));
+            for (uint32_t p = 0; p < d.S; ++p)
+                acc = acc.add(HostPt::load_xyzz(hwin + (wbeg[i] + (size_t)j * d.S + p) * sizeof(xyzz_t<F>)));
         }
         acc.store_jacobian(&out[i * 3 * sizeof(F)]);
     }
