@@ -211,7 +211,7 @@ def _pct(xs, q):
     return xs[min(len(xs) - 1, int(len(xs) * q))]
 
 
-def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5):
+def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=2):
     """Mixed-size request stream served two ways by one hzk_server holding all
     circuits' keys: continuous batching (mode 0) and the plain sequential
     prover through the same queue (mode 1). Requests arrive as a seeded Poisson
@@ -220,7 +220,7 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5):
     Both runs must return the same proof bytes request by request."""
     import threading
     import liblightzk_amd as z
-    srv = z.ProofServer(0, max_batch=max_batch, mode="sequential")
+    srv = z.ProofServer(0, max_batch=max_batch, mode="sequential", pipelines=pipelines)
     t = time.perf_counter()
     cids = [srv.add_circuit(40 + lg, lg, 8) for lg in logs]
     setup_s = time.perf_counter() - t
@@ -287,6 +287,7 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5):
     return {"circuits": [f"m=2^{lg}" for lg in logs], "weights": list(weights), "requests": requests,
             "arrivals": f"Poisson, seed {seed}, rate {rate:.1f}/s = {rate_mult}x plain capacity {capacity:.1f}/s",
             "plain_ms_per_circuit": [x * 1e3 for x in plain_t], "setup_s": setup_s, "max_batch": max_batch,
+            "pipelines": pipelines,
             "sequential": res["sequential"], "continuous": res["continuous"],
             "speedup_proofs_per_s": res["continuous"]["proofs_per_s"] / res["sequential"]["proofs_per_s"],
             "proofs_identical_across_modes": bool(same)}
@@ -307,6 +308,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate-mult", type=float, default=4.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--serve-pipelines", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
@@ -400,7 +402,8 @@ def main():
     if args.serve_requests > 0:
         out["groth16_served"] = groth16_served([int(v) for v in args.serve_logs.split(",")],
                                                [float(v) for v in args.serve_weights.split(",")],
-                                               args.serve_requests, args.serve_rate_mult, args.serve_max_batch)
+                                               args.serve_requests, args.serve_rate_mult, args.serve_max_batch,
+                                               pipelines=args.serve_pipelines)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scalars, d_b, args.cpu_seconds)
     print(json.dumps(out), flush=True)
