@@ -354,6 +354,12 @@ def main():
     acc_avg_s = acc_ms / max(acc_n, 1) * 1e-3
     work = st["entries"] * FQ_PER_MADD * INTMUL_PER_FQ
     achieved = work / acc_avg_s if acc_avg_s > 0 else 0.0
+    traffic = None   # HBM bytes per accumulate launch, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    prof = os.path.join(ROOT, "profiles", "msm_g1_2e24_hbm_traffic.txt")
+    if args.log_n == 24 and os.path.exists(prof):
+        for line in open(prof):
+            if line.startswith("k_msm_accumulate<Fq >"):
+                traffic = float(line.split()[-1]) * 1e9
     roofline = {
         "bound": "valu-intmul",
         "kernel": "msm_accumulate",
@@ -361,7 +367,8 @@ def main():
         "peak": peak_mad / 1e9,
         "unit": "G int-mul/s",
         "frac": achieved / peak_mad if peak_mad else None,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": "profiles/msm_g1_2e24_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)",
         "peak_source": "measured v_mad_u64_u32 issue rate on this GPU (hzk_roofline_probe)",
         "work_per_launch": work,
         "avg_launch_ms": acc_avg_s * 1e3,
