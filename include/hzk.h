@@ -79,6 +79,11 @@ int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, 
  * [X.c0|Y.c0|Z.c0][X.c1|Y.c1|Z.c1] (alt_bn128_t.cuh:244-256); level-major.
  * The MSM reads level 0 (the first n points) only. */
 #define HZK_BASES_REF_LEVELS 1u
+/* a fixed-base level table built by hzk_bases_levels for window size win_siz:
+ * L = ceil(255 / win_siz) affine copies, point j*n + i = 2^(win_siz*j) * P_i.
+ * All windows then share one bucket set (one bucket reduction instead of L),
+ * the layout the reference's pre_comp levels serve (msm.cuh:98-115). */
+#define HZK_BASES_AFFINE_LEVELS 2u
 int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
                size_t win_siz, void* dest);
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
@@ -95,6 +100,7 @@ typedef struct hzk_msm_desc {
     size_t n;
     void* dest;            /* Jacobian result (device, or host with HZK_BATCH_HOST_DEST) */
     uint32_t win_siz;      /* 0 = autotuned */
+    uint32_t levels;       /* 0, or ceil(255 / win_siz) when bases is an HZK_BASES_AFFINE_LEVELS table */
 } hzk_msm_desc;
 int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
 int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
@@ -161,6 +167,9 @@ int hzk_bases_convert(hzk_ctx* ctx, int group, uint32_t from_layout, uint32_t to
  * as the reference leaves it) and writes levels j = 1..level-1 as
  * 2^(gap*j) * P_i at record j*length + i (G2: point j*length + i) */
 int hzk_precomp(hzk_ctx* ctx, int group, void* points, size_t length, size_t gap, size_t level);
+/* HZK_BASES_AFFINE_LEVELS table for window size c from n affine bases:
+ * out[j*n + i] = 2^(c*j) * bases[i], j < ceil(255 / c) (device, affine) */
+int hzk_bases_levels(hzk_ctx* ctx, int group, int c, const void* bases, size_t n, void* out);
 
 /* ---- seeded synthetic inputs (replaces the reference's std::random_device draws,
  * include/host-arith/field/bigint.cuh:172-182, and Fp_model::random_element,

@@ -33,7 +33,7 @@ FIELD_BYTES = 32
 class MsmDesc(ctypes.Structure):
     """hzk_msm_desc: one instance of a batched MSM."""
     _fields_ = [("scalars", ctypes.c_void_p), ("bases", ctypes.c_void_p), ("n", ctypes.c_size_t),
-                ("dest", ctypes.c_void_p), ("win_siz", ctypes.c_uint32)]
+                ("dest", ctypes.c_void_p), ("win_siz", ctypes.c_uint32), ("levels", ctypes.c_uint32)]
 
 
 class NttDesc(ctypes.Structure):
@@ -202,10 +202,11 @@ class Context:
         self._check(load().hzk_coset_ntt(self.handle, int(coeffs), deg, 1 if inverse else 0))
 
     def msm_batch(self, group, jobs, host_dest=False, graph=False):
-        """jobs: list of (scalars, bases, n, dest[, win_siz]); device pointers (dest host if host_dest)."""
+        """jobs: list of (scalars, bases, n, dest[, win_siz[, levels]]); device pointers (dest host if
+        host_dest); levels != 0 marks bases as a level table built by bases_levels for win_siz."""
         arr = (MsmDesc * len(jobs))()
         for i, j in enumerate(jobs):
-            arr[i] = MsmDesc(int(j[0]), int(j[1]), j[2], int(j[3]), j[4] if len(j) > 4 else 0)
+            arr[i] = MsmDesc(int(j[0]), int(j[1]), j[2], int(j[3]), j[4] if len(j) > 4 else 0, j[5] if len(j) > 5 else 0)
         flags = (1 if host_dest else 0) | (2 if graph else 0)
         fn = load().hzk_msm_g1_batch if group == 1 else load().hzk_msm_g2_batch
         self._check(fn(self.handle, arr, len(jobs), flags))
@@ -228,11 +229,15 @@ class Context:
     def fixbase_g2(self, scalars, n, out_affine):
         self._check(load().hzk_fixbase_g2(self.handle, int(scalars), n, int(out_affine)))
 
-    def msm_g1(self, scalars, bases, n, dest, win_siz=0):
-        self._check(load().hzk_msm_g1(self.handle, int(scalars), int(bases), n, 0, win_siz, int(dest)))
+    def msm_g1(self, scalars, bases, n, dest, win_siz=0, layout=0):
+        self._check(load().hzk_msm_g1(self.handle, int(scalars), int(bases), n, layout, win_siz, int(dest)))
 
-    def msm_g2(self, scalars, bases, n, dest, win_siz=0):
-        self._check(load().hzk_msm_g2(self.handle, int(scalars), int(bases), n, 0, win_siz, int(dest)))
+    def bases_levels(self, group, c, bases, n, out):
+        """Level table out[j*n + i] = 2^(c j) bases[i] (affine), j < ceil(255 / c)."""
+        self._check(load().hzk_bases_levels(self.handle, group, c, int(bases), n, int(out)))
+
+    def msm_g2(self, scalars, bases, n, dest, win_siz=0, layout=0):
+        self._check(load().hzk_msm_g2(self.handle, int(scalars), int(bases), n, layout, win_siz, int(dest)))
 
 
 class Csr(ctypes.Structure):

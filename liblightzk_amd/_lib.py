@@ -60,6 +60,7 @@ def load():
         "hzk_fixbase_g2": (i32, [vp, vp, sz, vp]),
         "hzk_bases_convert": (i32, [vp, i32, u32, u32, vp, vp, sz]),
         "hzk_precomp": (i32, [vp, i32, vp, sz, sz, sz]),
+        "hzk_bases_levels": (i32, [vp, i32, i32, vp, sz, vp]),
         "hzk_device_sync": (i32, [vp]),
         "hzk_gen_field": (i32, [ctypes.c_uint64, ctypes.c_uint64, sz, i32, vp]),
         "hzk_spmv3": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),

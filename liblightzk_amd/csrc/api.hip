@@ -304,22 +304,30 @@ int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, 
 int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
                void* dest) {
     if (!ctx) return HZK_ERR_INVALID;
-    HZK_CHECK(ctx, base_layout <= HZK_BASES_REF_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g1: base layout %u unsupported", base_layout);
+    HZK_CHECK(ctx, base_layout <= HZK_BASES_AFFINE_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g1: base layout %u unsupported", base_layout);
     if (base_layout == HZK_BASES_REF_LEVELS) {
         HZK_CHECK(ctx, bases && n > 0, HZK_ERR_INVALID, "msm_g1: null bases or n = 0");
         int rc = ref_bases_affine(ctx, 1, bases, n, &bases);
         if (rc) return rc;
+    }
+    if (base_layout == HZK_BASES_AFFINE_LEVELS) {
+        HZK_CHECK(ctx, win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C, HZK_ERR_INVALID, "msm_g1: a level table needs its win_siz");
+        return msm_g1(ctx, scalars, bases, n, win_siz, dest, (uint32_t)((255 + win_siz - 1) / win_siz));
     }
     return msm_g1(ctx, scalars, bases, n, win_siz, dest);
 }
 int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout, size_t win_siz,
                void* dest) {
     if (!ctx) return HZK_ERR_INVALID;
-    HZK_CHECK(ctx, base_layout <= HZK_BASES_REF_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
+    HZK_CHECK(ctx, base_layout <= HZK_BASES_AFFINE_LEVELS, HZK_ERR_UNSUPPORTED, "msm_g2: base layout %u unsupported", base_layout);
     if (base_layout == HZK_BASES_REF_LEVELS) {
         HZK_CHECK(ctx, bases && n > 0, HZK_ERR_INVALID, "msm_g2: null bases or n = 0");
         int rc = ref_bases_affine(ctx, 2, bases, n, &bases);
         if (rc) return rc;
+    }
+    if (base_layout == HZK_BASES_AFFINE_LEVELS) {
+        HZK_CHECK(ctx, win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C, HZK_ERR_INVALID, "msm_g2: a level table needs its win_siz");
+        return msm_g2(ctx, scalars, bases, n, win_siz, dest, (uint32_t)((255 + win_siz - 1) / win_siz));
     }
     return msm_g2(ctx, scalars, bases, n, win_siz, dest);
 }
@@ -329,7 +337,7 @@ static int msm_batch_entry(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, ui
     HZK_CHECK(ctx, d && count > 0, HZK_ERR_INVALID, "msm batch: empty");
     HZK_CHECK(ctx, (flags & ~(HZK_BATCH_HOST_DEST | HZK_BATCH_GRAPH)) == 0, HZK_ERR_INVALID, "msm batch: bad flags %u", flags);
     std::vector<MsmJob> jobs(count);
-    for (size_t i = 0; i < count; ++i) jobs[i] = {d[i].scalars, d[i].bases, d[i].n, d[i].dest, d[i].win_siz};
+    for (size_t i = 0; i < count; ++i) jobs[i] = {d[i].scalars, d[i].bases, d[i].n, d[i].dest, d[i].win_siz, d[i].levels};
     int host = (flags & HZK_BATCH_HOST_DEST) ? 1 : 0, graph = (flags & HZK_BATCH_GRAPH) ? 1 : 0;
     return g2 ? msm_batch_g2(ctx, jobs.data(), count, host, graph) : msm_batch_g1(ctx, jobs.data(), count, host, graph);
 }

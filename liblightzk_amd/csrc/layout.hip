@@ -14,7 +14,8 @@
 // a reference table is read by converting its first `length` points into the
 // 64 / 128-byte affine layout. hzk_precomp rebuilds the reference's table in
 // place (level 0 normalised, levels 1.. by doubling), for callers that still
-// hand reference-layout tables around.
+// hand reference-layout tables around. hzk_bases_levels builds this library's
+// own level table (affine, level-major) for the single-bucket-set MSM.
 #include "common.hpp"
 #include "curve.hpp"
 
@@ -80,6 +81,21 @@ __global__ void __launch_bounds__(128) k_precomp(fq_t* __restrict__ rec, size_t 
     }
 }
 
+// fixed-base level table: out[j*n + i] = 2^(c j) in[i], affine, j < L
+template <class F>
+__global__ void __launch_bounds__(128) k_levels(const affine_t<F>* __restrict__ in, affine_t<F>* __restrict__ out, size_t n,
+                                                uint32_t c, uint32_t L) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const affine_t<F> a = in[i];
+    out[i] = a;
+    xyzz_t<F> cur = xyzz_t<F>::from_affine(a);
+    for (uint32_t j = 1; j < L; ++j) {
+        for (uint32_t k = 0; k < c; ++k) cur = cur.dbl();
+        out[(size_t)j * n + i] = to_affine_one(cur);
+    }
+}
+
 int convert_bases(hzk_ctx* ctx, int group, uint32_t from, uint32_t to, const void* src, void* dst, size_t n) {
     HZK_CHECK(ctx, group == 1 || group == 2, HZK_ERR_INVALID, "bases_convert: group %d", group);
     HZK_CHECK(ctx, src && dst && n > 0, HZK_ERR_INVALID, "bases_convert: null buffer or n = 0");
@@ -129,6 +145,22 @@ extern "C" {
 int hzk_bases_convert(hzk_ctx* ctx, int group, uint32_t from_layout, uint32_t to_layout, const void* src, void* dst, size_t n) {
     if (!ctx) return HZK_ERR_INVALID;
     return convert_bases(ctx, group, from_layout, to_layout, src, dst, n);
+}
+
+int hzk_bases_levels(hzk_ctx* ctx, int group, int c, const void* bases, size_t n, void* out) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, group == 1 || group == 2, HZK_ERR_INVALID, "bases_levels: group %d", group);
+    HZK_CHECK(ctx, c >= 4 && c <= 16 && bases && out && n > 0, HZK_ERR_INVALID, "bases_levels: c %d, n %zu", c, n);
+    const uint32_t L = (255 + c - 1) / c;
+    dim3 grid(blocks_for(n, 128)), block(128);
+    if (group == 1)
+        hipLaunchKernelGGL(k_levels<fq_t>, grid, block, 0, ctx->stream, (const affine_t<fq_t>*)bases, (affine_t<fq_t>*)out, n,
+                           (uint32_t)c, L);
+    else
+        hipLaunchKernelGGL(k_levels<fq2_t>, grid, block, 0, ctx->stream, (const affine_t<fq2_t>*)bases, (affine_t<fq2_t>*)out, n,
+                           (uint32_t)c, L);
+    HZK_HIP(ctx, hipGetLastError());
+    return HZK_OK;
 }
 
 int hzk_precomp(hzk_ctx* ctx, int group, void* points, size_t length, size_t gap, size_t level) {

@@ -48,6 +48,8 @@ struct MsmInst {
     const void* bases;     // affine points
     uint32_t n, c, W, NB, nk, K, g, G, T, tile;
     uint32_t S;            // window-sum workgroups per window
+    uint32_t nd, Wd;       // scalars and digit windows: (n, W), or for a level
+                           // table (n / Wd, Wd) with all Wd windows sharing one bucket set
 };
 
 struct MsmBatchDev {
@@ -79,8 +81,9 @@ constexpr uint32_t MSM_SCAN_TILE = 1024;
 // one-time kernel attribute setup (call outside graph capture)
 void msm_prepare();
 
-int msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest);
-int msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest);
+// levels != 0: bases is a level table (see MsmJob)
+int msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest, uint32_t levels = 0);
+int msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest, uint32_t levels = 0);
 
 // Batched MSM: out[i] (device or host pointer per dests_on_host) receives the
 // normalised Jacobian result of instance i. capture != 0 records the device
@@ -91,6 +94,7 @@ struct MsmJob {
     size_t n;
     void* dest;
     uint32_t win_siz;
+    uint32_t levels;   // 0: affine bases; else bases is a level table of levels = ceil(255 / win_siz) copies
 };
 int msm_batch_g1(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
 int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);

@@ -267,11 +267,31 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         d.G = p.NB / p.g;
         d.T = p.T;
         d.tile = p.tile;
+        d.nd = d.n;
+        d.Wd = d.W;
+        if (jobs[i].levels) {
+            // level table: n * W virtual points, c-bit digits, one window and
+            // one bucket set; a single window's reduction wants many short runs
+            d.nd = (uint32_t)jobs[i].n;
+            d.Wd = jobs[i].levels;
+            d.n = d.nd * d.Wd;
+            d.W = 1;
+            d.nk = d.NB;
+            uint32_t K = 4;
+            while (K < 4096 && (uint64_t)d.n / (2 * (uint64_t)K) >= ((uint64_t)1 << 18)) K <<= 1;
+            d.K = K;
+            d.g = d.NB >= 4 ? 2 : 1;
+            d.G = d.NB / d.g;
+            uint32_t T = 512;
+            while (T > 1 && (d.n / T < 4096 || d.n / T < 4 * d.NB)) T >>= 1;
+            d.T = T;
+            d.tile = (d.n + T - 1) / T;
+        }
         d.S = 1;   // window-sum partials: at most ~2 run sums per lane of 256
         while (d.S < 64 && d.G / d.S > 512) d.S *= 2;
         uint64_t entries = (uint64_t)d.n * d.W;
         uint64_t sz[SP_COUNT];
-        sz[SP_DIGIT] = d.n;
+        sz[SP_DIGIT] = d.nd;
         sz[SP_TILEWG] = (uint64_t)d.T * d.W;
         sz[SP_KEY] = (uint64_t)d.nk + 1;
         sz[SP_SCANTILE] = ((uint64_t)d.nk + MSM_SCAN_TILE) / MSM_SCAN_TILE;
@@ -366,7 +386,7 @@ static std::string shape_key(const MsmBatchLayout& L) {
     std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
     for (const MsmInst& d : L.inst) {
         char buf[64];
-        snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T);
+        snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T, d.Wd);
         k += buf;
     }
     return k;
@@ -377,6 +397,10 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     HZK_CHECK(ctx, count > 0 && jobs, HZK_ERR_INVALID, "msm: empty batch");
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].n > 0 && jobs[i].n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", jobs[i].n);
+        HZK_CHECK(ctx, !jobs[i].levels || (jobs[i].win_siz && jobs[i].levels == (255 + jobs[i].win_siz - 1) / jobs[i].win_siz),
+                  HZK_ERR_INVALID, "msm: a level table needs win_siz and ceil(255 / win_siz) = %u levels", jobs[i].levels);
+        HZK_CHECK(ctx, (uint64_t)jobs[i].n * (jobs[i].levels ? jobs[i].levels : 1) < (1ull << 31), HZK_ERR_UNSUPPORTED,
+                  "msm: n=%zu x %u levels exceeds 2^31 table points", jobs[i].n, jobs[i].levels);
         HZK_CHECK(ctx, jobs[i].scalars && jobs[i].bases && jobs[i].dest, HZK_ERR_INVALID, "msm: null pointer in job %zu", i);
         HZK_CHECK(ctx, jobs[i].win_siz == 0 || (jobs[i].win_siz >= (uint32_t)MSM_MIN_C && jobs[i].win_siz <= (uint32_t)MSM_MAX_C),
                   HZK_ERR_INVALID, "msm: win_siz=%u unsupported (0 or %d..%d)", jobs[i].win_siz, MSM_MIN_C, MSM_MAX_C);
@@ -465,10 +489,10 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
 }
 
 template <class F, class HostPt>
-int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest) {
+int msm_run(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, size_t win_siz, void* dest, uint32_t levels) {
     HZK_CHECK(ctx, win_siz <= (size_t)MSM_MAX_C, HZK_ERR_INVALID, "msm: win_siz=%zu unsupported (0 or %d..%d)", win_siz,
               MSM_MIN_C, MSM_MAX_C);
-    MsmJob j{scalars, bases, n, dest, (uint32_t)win_siz};
+    MsmJob j{scalars, bases, n, dest, (uint32_t)win_siz, levels};
     return msm_batch_run<F, HostPt>(ctx, &j, 1, 0, 0);
 }
 

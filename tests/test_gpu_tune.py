@@ -78,3 +78,30 @@ def test_autotune_then_exact(orc, data, tmp_path):
         assert tune.load(c3, path) == len(res)
         for lg, v in res.items():
             assert c3.msm_tune_get(1, lg) == tuple(v)
+
+
+@pytest.mark.parametrize("c", [6, 11, 13, 16])
+def test_level_tables_single_bucket_set(ctx, orc, data, c):
+    """MSM over a fixed-base level table (all windows in one bucket set) equals
+    the closed form for G1 and G2, uniform and sparse scalars, alone and in a
+    batch mixed with plain instances, with and without hipGraph."""
+    n, s, sp, k, b1, b2 = data
+    L = (255 + c - 1) // c
+    for group, bases, width in ((1, b1, 12), (2, b2, 24)):
+        db = ctx.upload(bases)
+        tab = ctx.alloc(bases.nbytes * L)
+        ctx.bases_levels(group, c, db, n, tab)
+        layout_levels = 2
+        for sc in (s, sp):
+            ds, d = ctx.upload(sc), ctx.alloc(8 * width)
+            (ctx.msm_g1 if group == 1 else ctx.msm_g2)(ds, tab, n, d, c, layout_levels)
+            assert np.array_equal(d.download(np.uint64, width), orc.msm_closed_form(sc, k, group)), (group, c)
+        ds = ctx.upload(s)
+        outs = [ctx.alloc(8 * width) for _ in range(3)]
+        jobs = [(ds, tab, n, outs[0], c, L), (ds, db, n, outs[1]), (ds, db, 1000, outs[2])]
+        for graph in (False, True, True):
+            ctx.msm_batch(group, jobs, graph=graph)
+            assert np.array_equal(outs[0].download(np.uint64, width), orc.msm_closed_form(s, k, group))
+            assert np.array_equal(outs[1].download(np.uint64, width), orc.msm_closed_form(s, k, group))
+            assert np.array_equal(outs[2].download(np.uint64, width),
+                                  orc.msm_closed_form(s[: 4 * 1000].copy(), k[: 4 * 1000].copy(), group))
