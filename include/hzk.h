@@ -79,10 +79,11 @@ int hzk_field_op(hzk_ctx* ctx, int field, int op, const void* a, const void* b, 
  * [X.c0|Y.c0|Z.c0][X.c1|Y.c1|Z.c1] (alt_bn128_t.cuh:244-256); level-major.
  * The MSM reads level 0 (the first n points) only. */
 #define HZK_BASES_REF_LEVELS 1u
-/* a fixed-base level table built by hzk_bases_levels for window size win_siz:
- * L = ceil(255 / win_siz) affine copies, point j*n + i = 2^(win_siz*j) * P_i.
- * All windows then share one bucket set (one bucket reduction instead of L),
- * the layout the reference's pre_comp levels serve (msm.cuh:98-115). */
+/* a fixed-base level table built by hzk_bases_levels for window size win_siz
+ * with all L = ceil(255 / win_siz) levels, point j*n + i = 2^(win_siz*j) * P_i:
+ * every window then shares one bucket set (one bucket reduction instead of L),
+ * the layout the reference's pre_comp levels serve (msm.cuh:98-115). Batched
+ * MSMs may use tables of fewer levels (hzk_msm_desc.levels). */
 #define HZK_BASES_AFFINE_LEVELS 2u
 int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, uint32_t base_layout,
                size_t win_siz, void* dest);
@@ -100,7 +101,8 @@ typedef struct hzk_msm_desc {
     size_t n;
     void* dest;            /* Jacobian result (device, or host with HZK_BATCH_HOST_DEST) */
     uint32_t win_siz;      /* 0 = autotuned */
-    uint32_t levels;       /* 0, or ceil(255 / win_siz) when bases is an HZK_BASES_AFFINE_LEVELS table */
+    uint32_t levels;       /* 0 or 1: affine bases; Lg > 1: bases is a level table of Lg levels
+                              (hzk_bases_levels with win_siz), each bucket set serving Lg windows */
 } hzk_msm_desc;
 int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
 int hzk_msm_g2_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags);
@@ -167,9 +169,9 @@ int hzk_bases_convert(hzk_ctx* ctx, int group, uint32_t from_layout, uint32_t to
  * as the reference leaves it) and writes levels j = 1..level-1 as
  * 2^(gap*j) * P_i at record j*length + i (G2: point j*length + i) */
 int hzk_precomp(hzk_ctx* ctx, int group, void* points, size_t length, size_t gap, size_t level);
-/* HZK_BASES_AFFINE_LEVELS table for window size c from n affine bases:
- * out[j*n + i] = 2^(c*j) * bases[i], j < ceil(255 / c) (device, affine) */
-int hzk_bases_levels(hzk_ctx* ctx, int group, int c, const void* bases, size_t n, void* out);
+/* level table for window size c from n affine bases: out[j*n + i] =
+ * 2^(c*j) * bases[i] for j < levels <= ceil(255 / c) (device, affine) */
+int hzk_bases_levels(hzk_ctx* ctx, int group, int c, int levels, const void* bases, size_t n, void* out);
 
 /* ---- seeded synthetic inputs (replaces the reference's std::random_device draws,
  * include/host-arith/field/bigint.cuh:172-182, and Fp_model::random_element,
@@ -194,6 +196,11 @@ int hzk_g16_witness(const hzk_g16* g, uint64_t wseed, void* z_host);
 int hzk_g16_rs(uint64_t wseed, void* rs_host);
 /* proof = A (G1 affine 64 B) | B (G2 affine 128 B) | C (G1 affine 64 B), host memory */
 int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_host, void* proof_host);
+/* prove over fixed-base level tables of the key with `levels` levels per
+ * bucket set (built on ctx; each MSM then reduces ceil(W / levels) bucket sets
+ * instead of W), or over the plain key arrays (levels <= 1, the default).
+ * Proofs are identical either way. */
+int hzk_g16_set_levels(hzk_ctx* ctx, hzk_g16* g, int levels);
 /* device pointer / count of a key array: 0 A1, 1 B1, 2 B2 (G2), 3 L1, 4 H1 */
 int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count);
 

@@ -232,9 +232,10 @@ class Context:
     def msm_g1(self, scalars, bases, n, dest, win_siz=0, layout=0):
         self._check(load().hzk_msm_g1(self.handle, int(scalars), int(bases), n, layout, win_siz, int(dest)))
 
-    def bases_levels(self, group, c, bases, n, out):
-        """Level table out[j*n + i] = 2^(c j) bases[i] (affine), j < ceil(255 / c)."""
-        self._check(load().hzk_bases_levels(self.handle, group, c, int(bases), n, int(out)))
+    def bases_levels(self, group, c, bases, n, out, levels=None):
+        """Level table out[j*n + i] = 2^(c j) bases[i] (affine), j < levels (default ceil(255 / c))."""
+        levels = levels or (255 + c - 1) // c
+        self._check(load().hzk_bases_levels(self.handle, group, c, levels, int(bases), n, int(out)))
 
     def msm_g2(self, scalars, bases, n, dest, win_siz=0, layout=0):
         self._check(load().hzk_msm_g2(self.handle, int(scalars), int(bases), n, layout, win_siz, int(dest)))
@@ -301,6 +302,10 @@ class Groth16:
         proof = np.empty(32, dtype=np.uint64)
         self.ctx._check(load().hzk_g16_prove(self.ctx.handle, self.h, z.ctypes.data, rs.ctypes.data, proof.ctypes.data))
         return proof
+
+    def set_levels(self, levels):
+        """Prove over level tables with `levels` levels per bucket set (<= 1: plain key arrays)."""
+        self.ctx._check(load().hzk_g16_set_levels(self.ctx.handle, self.h, levels))
 
     def key_array(self, which):
         p, n = ctypes.c_void_p(), ctypes.c_size_t()

@@ -60,7 +60,7 @@ def load():
         "hzk_fixbase_g2": (i32, [vp, vp, sz, vp]),
         "hzk_bases_convert": (i32, [vp, i32, u32, u32, vp, vp, sz]),
         "hzk_precomp": (i32, [vp, i32, vp, sz, sz, sz]),
-        "hzk_bases_levels": (i32, [vp, i32, i32, vp, sz, vp]),
+        "hzk_bases_levels": (i32, [vp, i32, i32, i32, vp, sz, vp]),
         "hzk_device_sync": (i32, [vp]),
         "hzk_gen_field": (i32, [ctypes.c_uint64, ctypes.c_uint64, sz, i32, vp]),
         "hzk_spmv3": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -72,6 +72,7 @@ def load():
         "hzk_g16_rs": (i32, [ctypes.c_uint64, vp]),
         "hzk_g16_prove": (i32, [vp, vp, vp, vp, vp]),
         "hzk_g16_key_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
+        "hzk_g16_set_levels": (i32, [vp, vp, i32]),
         "hzk_server_create": (i32, [i32, sz, i32, i32, ctypes.POINTER(vp)]),
         "hzk_server_destroy": (None, [vp]),
         "hzk_server_set_pipelines": (i32, [vp, i32]),

@@ -311,7 +311,7 @@ int hzk_msm_g1(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
         if (rc) return rc;
     }
     if (base_layout == HZK_BASES_AFFINE_LEVELS) {
-        HZK_CHECK(ctx, win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C, HZK_ERR_INVALID, "msm_g1: a level table needs its win_siz");
+        HZK_CHECK(ctx, win_siz >= MSM_MIN_C && win_siz <= MSM_MAX_C, HZK_ERR_INVALID, "msm_g1: a level table needs its win_siz");   // full table: ceil(255 / c) levels
         return msm_g1(ctx, scalars, bases, n, win_siz, dest, (uint32_t)((255 + win_siz - 1) / win_siz));
     }
     return msm_g1(ctx, scalars, bases, n, win_siz, dest);

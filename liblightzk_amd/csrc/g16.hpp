@@ -29,6 +29,11 @@ struct hzk_g16 {
     std::vector<hzk::host::G1x> IC;   // verifying-key input commitments
     // buffers of the plain one-proof-at-a-time prover, and 1/(g^M - 1)
     void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *zinv = nullptr;
+    // fixed-base level tables of the key arrays for window size lc with lv
+    // levels (lv <= 1: none); each MSM bucket set then serves lv windows
+    void *A1L = nullptr, *B1L = nullptr, *B2L = nullptr, *L1L = nullptr, *H1L = nullptr;
+    int lc = 0;
+    uint32_t lv = 1;
     std::vector<void*> owned;
     int device = 0;
 };

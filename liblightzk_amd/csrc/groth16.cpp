@@ -24,6 +24,7 @@
 // Streams: 2 circuit structure, 3 witness inputs, 4 toxic waste
 // (tau, alpha, beta, gamma, delta), 5 prover randomness (r, s).
 #include <omp.h>
+#include <algorithm>
 #include <thread>
 #include <vector>
 #include "common.hpp"
@@ -140,6 +141,51 @@ int key_points(hzk_ctx* ctx, hzk_g16* g, const std::vector<Fr>& k, void** out) {
     return rc;
 }
 
+// Window size of the key's level tables: minimises bucket entries
+// n * ceil(255 / c) plus the reduction of its bucket sets, 3 * 2^(c-1) each.
+int level_window(size_t n, uint32_t levels) {
+    int best = 16;
+    double bc = -1;
+    for (int c = MSM_MIN_C; c <= 16; ++c) {
+        const uint32_t W = (255 + c - 1) / c, sets = (W + levels - 1) / levels;
+        double cost = (double)n * W + 3.0 * sets * (double)(1u << (c - 1));
+        if (bc < 0 || cost < bc) { bc = cost; best = c; }
+    }
+    return best;
+}
+
+// Level tables of the five key arrays with `levels` levels each (point
+// t*n + i = 2^(c t) P_i): a proof's MSMs then reduce ceil(W / levels) bucket
+// sets each instead of W. levels <= 1 frees them (plain key arrays).
+int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
+    for (void** p : {&g->A1L, &g->B1L, &g->L1L, &g->H1L, &g->B2L}) {
+        if (*p) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(*p);
+            g->owned.erase(std::remove(g->owned.begin(), g->owned.end(), *p), g->owned.end());
+            *p = nullptr;
+        }
+    }
+    g->lv = 1;
+    g->lc = 0;
+    if (levels <= 1) return 0;
+    const size_t N1 = g->num_vars, Ln = N1 - g->num_public - 1, Hn = g->M - 1;
+    const int c = level_window(N1, levels);
+    const uint32_t lv = std::min<uint32_t>(levels, (uint32_t)((255 + c - 1) / c));
+    struct T { void* src; size_t n; int group; void** dst; };
+    const T tabs[5] = {{g->A1, N1, 1, &g->A1L}, {g->B1, N1, 1, &g->B1L}, {g->L1, Ln, 1, &g->L1L}, {g->H1, Hn, 1, &g->H1L},
+                       {g->B2, N1, 2, &g->B2L}};
+    for (const T& t : tabs) {
+        int rc = dev_alloc(ctx, g, t.n * lv * (t.group == 1 ? 64 : 128), t.dst);
+        if (!rc) rc = hzk_bases_levels(ctx, t.group, c, (int)lv, t.src, t.n, *t.dst);
+        if (rc) return rc;
+    }
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    g->lc = c;
+    g->lv = lv;
+    return 0;
+}
+
 int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
     build_circuit(g, seed);
     // device CSR
@@ -245,11 +291,20 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
         hzk_g16* g = reqs[i].g;
         const size_t N1 = g->num_vars, l = g->num_public;
         char* r = reqs[i].res;
-        j1.push_back({reqs[i].z, g->A1, N1, r, 0});
-        j1.push_back({reqs[i].z, g->B1, N1, r + 96, 0});
-        j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1, N1 - l - 1, r + 384, 0});
-        j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0});
-        j2.push_back({reqs[i].z, g->B2, N1, r + 192, 0});
+        if (g->lv > 1) {
+            const uint32_t c = (uint32_t)g->lc, L = g->lv;
+            j1.push_back({reqs[i].z, g->A1L, N1, r, c, L});
+            j1.push_back({reqs[i].z, g->B1L, N1, r + 96, c, L});
+            j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1L, N1 - l - 1, r + 384, c, L});
+            j1.push_back({reqs[i].a, g->H1L, g->M - 1, r + 480, c, L});
+            j2.push_back({reqs[i].z, g->B2L, N1, r + 192, c, L});
+        } else {
+            j1.push_back({reqs[i].z, g->A1, N1, r, 0, 0});
+            j1.push_back({reqs[i].z, g->B1, N1, r + 96, 0, 0});
+            j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1, N1 - l - 1, r + 384, 0, 0});
+            j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0, 0});
+            j2.push_back({reqs[i].z, g->B2, N1, r + 192, 0, 0});
+        }
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with
     // the G1 batch: the latency-bound tails of one (bucket reduction, window
@@ -361,6 +416,7 @@ void hzk_g16_free(hzk_g16* g) {
 int hzk_g16_info(const hzk_g16* g, uint64_t* out8) {
     if (!g || !out8) return HZK_ERR_INVALID;
     uint64_t kb = g->num_vars * (64 + 64 + 128) + (g->num_vars - g->num_public - 1) * 64 + (g->M - 1) * 64;
+    kb *= g->lv > 1 ? 1 + g->lv : 1;   // plus the level tables
     uint64_t v[8] = {g->M, g->num_vars, g->num_public, g->num_gates, g->m[0].col.size(), g->m[1].col.size(),
                      g->m[2].col.size(), kb};
     std::memcpy(out8, v, sizeof v);
@@ -397,6 +453,14 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
     if (rc) return rc;
     g16_assemble(g, r.res, rs_host, proof_host);
     return HZK_OK;
+}
+
+// Prove over fixed-base level tables with `levels` levels per bucket set
+// (built here, on ctx), or over the plain key arrays (levels <= 1).
+int hzk_g16_set_levels(hzk_ctx* ctx, hzk_g16* g, int levels) {
+    if (!ctx || !g) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, levels >= 0 && levels <= 64, HZK_ERR_INVALID, "g16_set_levels: %d", levels);
+    return build_levels(ctx, g, (uint32_t)levels);
 }
 
 // Device pointer and element count of a proving-key array (tests, serving):

@@ -147,11 +147,12 @@ int hzk_bases_convert(hzk_ctx* ctx, int group, uint32_t from_layout, uint32_t to
     return convert_bases(ctx, group, from_layout, to_layout, src, dst, n);
 }
 
-int hzk_bases_levels(hzk_ctx* ctx, int group, int c, const void* bases, size_t n, void* out) {
+int hzk_bases_levels(hzk_ctx* ctx, int group, int c, int levels, const void* bases, size_t n, void* out) {
     if (!ctx) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, group == 1 || group == 2, HZK_ERR_INVALID, "bases_levels: group %d", group);
     HZK_CHECK(ctx, c >= 4 && c <= 16 && bases && out && n > 0, HZK_ERR_INVALID, "bases_levels: c %d, n %zu", c, n);
-    const uint32_t L = (255 + c - 1) / c;
+    HZK_CHECK(ctx, levels >= 1 && levels <= (255 + c - 1) / c, HZK_ERR_INVALID, "bases_levels: %d levels for c %d", levels, c);
+    const uint32_t L = (uint32_t)levels;
     dim3 grid(blocks_for(n, 128)), block(128);
     if (group == 1)
         hipLaunchKernelGGL(k_levels<fq_t>, grid, block, 0, ctx->stream, (const affine_t<fq_t>*)bases, (affine_t<fq_t>*)out, n,

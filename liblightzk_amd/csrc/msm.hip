@@ -59,8 +59,9 @@ HZK_DEV const uint64_t* space(const MsmBatchDev& b, int s) { return b.begin + (s
 
 // ------------------------------------------------------------------ 1. digits
 // digits[j * n + i] = |d_ij| | sign << 31, or 0 for a zero digit. For a level
-// table the same array is the single window of n * W virtual points
-// 2^(c j) P_i, so every window lands in one bucket set.
+// table of Lg levels the same array is, window by window of the pipeline, the
+// digits of Lg consecutive digit windows over nd * Lg virtual points
+// 2^(c (j mod Lg)) P_i, so Lg digit windows share one bucket set.
 __global__ void __launch_bounds__(256) k_msm_digits(MsmBatchDev b, uint32_t* __restrict__ digits) {
     __shared__ uint32_t limbs[256][9];
     const uint64_t* beg = space(b, SP_DIGIT);
@@ -77,15 +78,17 @@ __global__ void __launch_bounds__(256) k_msm_digits(MsmBatchDev b, uint32_t* __r
         const uint32_t c = d.c, NB = d.NB, mask = (1u << c) - 1;
         uint32_t carry = 0;
         uint32_t* out = digits + ebeg[ii] + idx;
-        for (uint32_t j = 0; j < d.Wd; ++j) {
-            uint32_t lo = j * c, w = lo >> 5, sh = lo & 31;
-            uint64_t v = (uint64_t)my[w] | ((uint64_t)my[w + 1] << 32);
-            uint32_t raw = ((uint32_t)(v >> sh) & mask) + carry;
-            uint32_t dg;
-            if (raw > NB) { carry = 1; dg = (2 * NB - raw) | DIGIT_NEG; }
-            else { carry = 0; dg = raw; }
-            if ((dg & ~DIGIT_NEG) == 0) dg = 0;
-            out[(size_t)j * d.nd] = dg;   // window j of scalar idx (level tables: point j * nd + idx)
+        for (uint32_t j = 0; j < d.W * d.Lg; ++j) {   // level tables: windows past Wd pad the last set
+            uint32_t dg = 0;
+            if (j < d.Wd) {
+                uint32_t lo = j * c, w = lo >> 5, sh = lo & 31;
+                uint64_t v = (uint64_t)my[w] | ((uint64_t)my[w + 1] << 32);
+                uint32_t raw = ((uint32_t)(v >> sh) & mask) + carry;
+                if (raw > NB) { carry = 1; dg = (2 * NB - raw) | DIGIT_NEG; }
+                else { carry = 0; dg = raw; }
+                if ((dg & ~DIGIT_NEG) == 0) dg = 0;
+            }
+            out[(size_t)j * d.nd] = dg;   // window j of scalar idx (level tables: set j / Lg, point (j % Lg) nd + idx)
         }
     }
 }

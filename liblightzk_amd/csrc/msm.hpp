@@ -48,8 +48,11 @@ struct MsmInst {
     const void* bases;     // affine points
     uint32_t n, c, W, NB, nk, K, g, G, T, tile;
     uint32_t S;            // window-sum workgroups per window
-    uint32_t nd, Wd;       // scalars and digit windows: (n, W), or for a level
-                           // table (n / Wd, Wd) with all Wd windows sharing one bucket set
+    // Level tables: nd scalars with Wd digit windows over a table of Lg levels
+    // (point t*nd + i = 2^(c t) P_i, t < Lg); each bucket set ("window" of the
+    // pipeline) serves Lg consecutive digit windows, so W = ceil(Wd / Lg) and
+    // n = nd * Lg virtual points. Plain bases: nd = n, Wd = W, Lg = 1.
+    uint32_t nd, Wd, Lg;
 };
 
 struct MsmBatchDev {
@@ -94,7 +97,7 @@ struct MsmJob {
     size_t n;
     void* dest;
     uint32_t win_siz;
-    uint32_t levels;   // 0: affine bases; else bases is a level table of levels = ceil(255 / win_siz) copies
+    uint32_t levels;   // 0 or 1: affine bases; else levels per bucket set of a level table (<= ceil(255 / win_siz))
 };
 int msm_batch_g1(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
 int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);

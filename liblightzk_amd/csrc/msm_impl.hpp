@@ -269,20 +269,19 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         d.tile = p.tile;
         d.nd = d.n;
         d.Wd = d.W;
-        if (jobs[i].levels) {
-            // level table: n * W virtual points, c-bit digits, one window and
-            // one bucket set; a single window's reduction wants many short runs
-            d.nd = (uint32_t)jobs[i].n;
-            d.Wd = jobs[i].levels;
-            d.n = d.nd * d.Wd;
-            d.W = 1;
-            d.nk = d.NB;
+        d.Lg = 1;
+        if (jobs[i].levels > 1) {
+            // level table: every bucket set serves Lg digit windows over nd * Lg
+            // virtual points; fewer bucket sets to reduce, larger buckets
+            d.Lg = jobs[i].levels;
+            d.W = (d.Wd + d.Lg - 1) / d.Lg;
+            d.n = d.nd * d.Lg;
+            d.nk = d.W * d.NB;
+            const uint64_t entries = (uint64_t)d.n * d.W;
             uint32_t K = 4;
-            while (K < 4096 && (uint64_t)d.n / (2 * (uint64_t)K) >= ((uint64_t)1 << 18)) K <<= 1;
+            while (K < 4096 && entries / (2 * (uint64_t)K) >= ((uint64_t)1 << 18)) K <<= 1;
             d.K = K;
-            d.g = d.NB >= 4 ? 2 : 1;
-            d.G = d.NB / d.g;
-            uint32_t T = 512;
+            uint32_t T = (512 + d.W - 1) / d.W;
             while (T > 1 && (d.n / T < 4096 || d.n / T < 4 * d.NB)) T >>= 1;
             d.T = T;
             d.tile = (d.n + T - 1) / T;
@@ -386,7 +385,7 @@ static std::string shape_key(const MsmBatchLayout& L) {
     std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
     for (const MsmInst& d : L.inst) {
         char buf[64];
-        snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T, d.Wd);
+        snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T, d.Lg);
         k += buf;
     }
     return k;
@@ -397,8 +396,8 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     HZK_CHECK(ctx, count > 0 && jobs, HZK_ERR_INVALID, "msm: empty batch");
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].n > 0 && jobs[i].n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", jobs[i].n);
-        HZK_CHECK(ctx, !jobs[i].levels || (jobs[i].win_siz && jobs[i].levels == (255 + jobs[i].win_siz - 1) / jobs[i].win_siz),
-                  HZK_ERR_INVALID, "msm: a level table needs win_siz and ceil(255 / win_siz) = %u levels", jobs[i].levels);
+        HZK_CHECK(ctx, jobs[i].levels <= 1 || (jobs[i].win_siz && jobs[i].levels <= (255 + jobs[i].win_siz - 1) / jobs[i].win_siz),
+                  HZK_ERR_INVALID, "msm: a level table needs win_siz and at most ceil(255 / win_siz) levels (%u)", jobs[i].levels);
         HZK_CHECK(ctx, (uint64_t)jobs[i].n * (jobs[i].levels ? jobs[i].levels : 1) < (1ull << 31), HZK_ERR_UNSUPPORTED,
                   "msm: n=%zu x %u levels exceeds 2^31 table points", jobs[i].n, jobs[i].levels);
         HZK_CHECK(ctx, jobs[i].scalars && jobs[i].bases && jobs[i].dest, HZK_ERR_INVALID, "msm: null pointer in job %zu", i);
@@ -468,7 +467,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         const MsmInst& d = L.inst[i];
         HostPt acc = HostPt::inf();
         for (int j = (int)d.W - 1; j >= 0; --j) {
-            for (uint32_t q = 0; q < d.c; ++q) acc = acc.dbl();
+            for (uint32_t q = 0; q < d.c * d.Lg; ++q) acc = acc.dbl();   // a bucket set spans Lg digit windows
             for (uint32_t p = 0; p < d.S; ++p)
                 acc = acc.add(HostPt::load_xyzz(hwin + (wbeg[i] + (size_t)j * d.S + p) * sizeof(xyzz_t<F>)));
         }

@@ -55,6 +55,9 @@ def test_proof_bit_exact_and_verifies(pair):
             assert np.array_equal(p, o.prove(z, rs)), wseed
         assert o.verify(z[: 4 * 9], p)
         assert np.array_equal(p, g.prove(z, rs))   # deterministic
+        for lv in (4, 16, 1):                        # level tables: bucket sets shared by lv windows
+            g.set_levels(lv)
+            assert np.array_equal(p, g.prove(z, rs)), lv
     bad = p.copy()
     bad[0] ^= np.uint64(1)
     assert not o.verify(z[: 4 * 9], bad)

@@ -99,8 +99,13 @@ def test_level_tables_single_bucket_set(ctx, orc, data, c):
         ds = ctx.upload(s)
         outs = [ctx.alloc(8 * width) for _ in range(3)]
         jobs = [(ds, tab, n, outs[0], c, L), (ds, db, n, outs[1]), (ds, db, 1000, outs[2])]
+        part = ctx.alloc(bases.nbytes * 3)          # 3-level table: bucket sets of 3 windows
+        ctx.bases_levels(group, c, db, n, part, levels=3)
+        jobs.append((ds, part, n, ctx.alloc(8 * width), c, 3))
+        outs.append(jobs[-1][3])
         for graph in (False, True, True):
             ctx.msm_batch(group, jobs, graph=graph)
+            assert np.array_equal(outs[3].download(np.uint64, width), orc.msm_closed_form(s, k, group))
             assert np.array_equal(outs[0].download(np.uint64, width), orc.msm_closed_form(s, k, group))
             assert np.array_equal(outs[1].download(np.uint64, width), orc.msm_closed_form(s, k, group))
             assert np.array_equal(outs[2].download(np.uint64, width),
