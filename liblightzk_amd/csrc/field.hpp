@@ -315,10 +315,10 @@ struct fp {
 typedef fp<FqParams> fq_t;
 typedef fp<FrParams> fr_t;
 
-// Fq2 = Fq[u]/(u^2 + 1), one element per lane (no cross-lane traffic). A G2
-// point operation would inline ~30 Fq products and overflow the instruction
-// cache, so the Fq2 product and square are out-of-line calls, each running its
-// Fq products as interleaved chains.
+// Fq2 = Fq[u]/(u^2 + 1), one element per lane (no cross-lane traffic). The
+// Fq2 product and square are inlined: as out-of-line calls they cost every
+// caller a stack frame of saved registers (432 bytes of scratch per lane in
+// the G2 accumulation) and measured 19 % slower than the larger inlined loop.
 struct fq2_t {
     fq_t c0, c1;
     static HZK_DEV fq2_t zero() { return {fq_t::zero(), fq_t::zero()}; }
@@ -330,14 +330,14 @@ struct fq2_t {
     HZK_DEV fq2_t dbl() const { return {c0.dbl(), c1.dbl()}; }
     HZK_DEV fq2_t neg() const { return {c0.neg(), c1.neg()}; }
     HZK_DEV fq2_t cneg(bool f) const { return f ? neg() : *this; }
-    // Karatsuba: 3 Fq multiplications, interleaved in one call
-    static __device__ __noinline__ fq2_t mul_call(fq2_t a, fq2_t b) {
+    // Karatsuba: 3 Fq multiplications, interleaved
+    static HZK_DEV fq2_t mul_call(fq2_t a, fq2_t b) {
         fq_t aa, bb, s;
         fq_t::mul3(aa, a.c0, b.c0, bb, a.c1, b.c1, s, a.c0 + a.c1, b.c0 + b.c1);
         return {aa - bb, s - aa - bb};
     }
-    // complex squaring: 2 Fq multiplications, interleaved in one call
-    static __device__ __noinline__ fq2_t sqr_call(fq2_t a) {
+    // complex squaring: 2 Fq multiplications, interleaved
+    static HZK_DEV fq2_t sqr_call(fq2_t a) {
         fq_t ab, t;
         fq_t::mul2(ab, a.c0, a.c1, t, a.c0 + a.c1, a.c0 - a.c1);
         return {t, ab.dbl()};
