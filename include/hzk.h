@@ -264,8 +264,9 @@ int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value);
  * A plan per group (1 = G1, 2 = G2) and size class log_n = floor(log2 n):
  * window bits c, accumulation chunk length K (entries per lane, a multiple of
  * 4) and bucket-reduction run length g (a power of two <= 2^(c-1)). Every MSM
- * on the context, single or batched, uses the class's plan; results do not
- * depend on the plan. c = 0 clears the entry (built-in heuristic). */
+ * on the context, single or in a narrow batch, uses the class's plan; results
+ * do not depend on the plan. c = 0 clears the entry (built-in table or
+ * heuristic). */
 int hzk_msm_tune_set(hzk_ctx* ctx, int group, int log_n, int c, uint32_t K, uint32_t g);
 /* the plan an MSM of 2^log_n points would use now */
 int hzk_msm_tune_get(hzk_ctx* ctx, int group, int log_n, int* c, uint32_t* K, uint32_t* g);
@@ -273,10 +274,20 @@ int hzk_msm_tune_get(hzk_ctx* ctx, int group, int log_n, int* c, uint32_t* K, ui
  * inputs (coordinate descent over c, K, g; at most budget_ms per class) and
  * store the fastest in the context's table */
 int hzk_msm_autotune(hzk_ctx* ctx, int group, int log_min, int log_max, double budget_ms);
+/* The same for the throughput regime: the plans used by the instances of a
+ * batch of two or more MSMs whose digit count (under the single-MSM plans)
+ * reaches 2^26, which fills the GPU without help from the plan; typically
+ * fewer window bits (fewer buckets to reduce), longer chunks and runs. The
+ * autotuner times each class as a batch of 2^26 / (16 n) instances (2..256). */
+int hzk_msm_batch_tune_set(hzk_ctx* ctx, int group, int log_n, int c, uint32_t K, uint32_t g);
+int hzk_msm_batch_tune_get(hzk_ctx* ctx, int group, int log_n, int* c, uint32_t* K, uint32_t* g);
+int hzk_msm_batch_autotune(hzk_ctx* ctx, int group, int log_min, int log_max, double budget_ms);
 
 /* ---- tuning / test hooks ---- */
 /* force MSM window bits and bucket-reduction segment size (0 = automatic) */
 int hzk_set_tuning(hzk_ctx* ctx, int window_bits, int segment);
+/* MSM plan regime: -1 by batch width (default), 0 single-MSM plans, 1 wide-batch plans */
+int hzk_set_msm_regime(hzk_ctx* ctx, int regime);
 /* make the n-th following scratch allocation fail with HZK_ERR_NOMEM (-1 = off) */
 int hzk_inject_alloc_failure(hzk_ctx* ctx, long after_n);
 

@@ -172,6 +172,10 @@ class Context:
     def set_tuning(self, window_bits=0, segment=0):
         self._check(load().hzk_set_tuning(self.handle, window_bits, segment))
 
+    def set_msm_regime(self, regime):
+        """-1 by batch width (default), 0 single-MSM plans, 1 wide-batch plans."""
+        self._check(load().hzk_set_msm_regime(self.handle, regime))
+
     def msm_tune_set(self, group, log_n, c, K=0, g=0):
         self._check(load().hzk_msm_tune_set(self.handle, group, log_n, c, K, g))
 
@@ -183,6 +187,19 @@ class Context:
     def msm_autotune(self, group, log_min, log_max, budget_ms=2000.0):
         self._check(load().hzk_msm_autotune(self.handle, group, log_min, log_max, float(budget_ms)))
         return {lg: self.msm_tune_get(group, lg) for lg in range(log_min, log_max + 1)}
+
+    def msm_batch_tune_set(self, group, log_n, c, K=0, g=0):
+        """Plan of a size class in the wide-batch (throughput) regime."""
+        self._check(load().hzk_msm_batch_tune_set(self.handle, group, log_n, c, K, g))
+
+    def msm_batch_tune_get(self, group, log_n):
+        c, K, g = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(load().hzk_msm_batch_tune_get(self.handle, group, log_n, ctypes.byref(c), ctypes.byref(K), ctypes.byref(g)))
+        return c.value, K.value, g.value
+
+    def msm_batch_autotune(self, group, log_min, log_max, budget_ms=2000.0):
+        self._check(load().hzk_msm_batch_autotune(self.handle, group, log_min, log_max, float(budget_ms)))
+        return {lg: self.msm_batch_tune_get(group, lg) for lg in range(log_min, log_max + 1)}
 
     def inject_alloc_failure(self, after_n):
         self._check(load().hzk_inject_alloc_failure(self.handle, after_n))

@@ -91,9 +91,13 @@ def load():
         "hzk_timing_get": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(dbl), ctypes.POINTER(i64)]),
         "hzk_timing_report": (sz, [vp, ctypes.c_char_p, sz]),
         "hzk_set_tuning": (i32, [vp, i32, i32]),
+        "hzk_set_msm_regime": (i32, [vp, i32]),
         "hzk_msm_tune_set": (i32, [vp, i32, i32, i32, u32, u32]),
         "hzk_msm_tune_get": (i32, [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "hzk_msm_autotune": (i32, [vp, i32, i32, i32, dbl]),
+        "hzk_msm_batch_tune_set": (i32, [vp, i32, i32, i32, u32, u32]),
+        "hzk_msm_batch_tune_get": (i32, [vp, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "hzk_msm_batch_autotune": (i32, [vp, i32, i32, i32, dbl]),
         "hzk_roofline_probe": (i32, [vp, i32, ctypes.POINTER(dbl)]),
         "hzk_inject_alloc_failure": (i32, [vp, i64]),
     }

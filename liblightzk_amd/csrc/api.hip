@@ -403,6 +403,12 @@ int hzk_set_tuning(hzk_ctx* ctx, int window_bits, int segment) {
     ctx->force_seg = segment;
     return HZK_OK;
 }
+int hzk_set_msm_regime(hzk_ctx* ctx, int regime) {
+    if (!ctx) return HZK_ERR_INVALID;
+    HZK_CHECK(ctx, regime >= -1 && regime <= 1, HZK_ERR_INVALID, "msm regime %d (-1 auto, 0 single, 1 wide batch)", regime);
+    ctx->force_regime = regime;
+    return HZK_OK;
+}
 int hzk_inject_alloc_failure(hzk_ctx* ctx, long after_n) {
     if (!ctx) return HZK_ERR_INVALID;
     ctx->fail_alloc_after = after_n;

@@ -62,7 +62,11 @@ struct hzk_ctx {
     uint32_t last_msm[5] = {0, 0, 0, 0, 0};
     // NTT twiddle / coset tables for the last transform size (ntt.hip)
     void* ntt_cache = nullptr;
-    hzk_tune_entry tune[2][32];   // [G1, G2][log2 n]
+    // MSM plans [regime][G1, G2][log2 n]; regime 0: an MSM that has the GPU
+    // to itself (latency), 1: an instance of a batch wide enough to fill the
+    // GPU by itself (throughput)
+    hzk_tune_entry tune[2][2][32];
+    int force_regime = -1;        // -1: by batch width; 0 / 1: forced (autotuner)
     // second context (own stream and scratch) for work overlapped with this one's
     hzk_ctx* aux = nullptr;
     // autotune overrides (0 = automatic)

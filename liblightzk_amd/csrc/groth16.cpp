@@ -332,14 +332,22 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
 
 // Upper bound of one MSM's scratch (digits and sorted entries, histograms,
 // key scans, bucket sums, chunk partials, bucket-reduction and window sums)
-// for n points with xyzz points of pt_bytes, from the built-in plan.
+// for n points with xyzz points of pt_bytes, from the built-in plans (the
+// larger of the single-MSM and wide-batch regimes).
 static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
-    MsmPlan p = msm_plan(n, 0, 0, 160 * 1024);
-    size_t W = p.W, nk = W * p.NB, entries = n * W;
-    size_t chunks = (entries + p.K - 1) / p.K;
-    size_t bred = W * ((p.NB + p.g - 1) / p.g);
-    return entries * 8 + (size_t)p.T * nk * 4 + 2 * (nk + 1) * 4 + 2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 +
-           (nk + 1) * pt_bytes + chunks * (4 + 2 * pt_bytes) + (bred + W) * pt_bytes + 4096;
+    size_t worst = 0;
+    for (int regime : {MSM_REGIME_SINGLE, MSM_REGIME_BATCH}) {
+        const MsmPlan p = msm_plan_builtin(pt_bytes > 128, n, regime, 160 * 1024);
+        size_t W = p.W, nk = W * p.NB, entries = n * W;
+        size_t chunks = (entries + p.K - 1) / p.K;
+        size_t bred = W * ((p.NB + p.g - 1) / p.g);
+        const size_t coarse = p.NB >= 2048 && n >= MSM_TP_BATCH_MIN_N ? entries * 8 : 0;   // two-pass staging
+        size_t b = entries * 8 + coarse + (size_t)p.T * nk * 4 + 2 * (nk + 1) * 4 +
+                   2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 + (nk + 1) * pt_bytes + chunks * (4 + 2 * pt_bytes) +
+                   (bred + W) * pt_bytes + 4096;
+        worst = std::max(worst, b);
+    }
+    return worst;
 }
 
 size_t g16_request_footprint(const hzk_g16* g) {

@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(1024) k_msm_scatter(MsmBatchDev b, const uint3
     const uint64_t* beg = space(b, SP_TILEWG);
     uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (msm_two_pass(d.n, d.NB)) return;   // scattered in two passes below
+    if (d.tp) return;   // scattered in two passes below
     uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     uint32_t t = local % d.T, j = local / d.T;
     const uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_coarse(MsmBatchDev b
     const uint64_t* beg = space(b, SP_TILEWG);
     const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (!msm_two_pass(d.n, d.NB)) return;
+    if (!d.tp) return;
     const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     const uint32_t t = local % d.T, j = local / d.T;
     const uint32_t FB = d.NB / MSM_COARSE, fshift = __ffs(FB) - 1;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_fine(MsmBatchDev b, 
     const uint64_t* beg = space(b, SP_CGROUP);
     const uint32_t ii = find_inst(beg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
-    if (!msm_two_pass(d.n, d.NB)) return;
+    if (!d.tp) return;
     const uint32_t local = (uint32_t)(blockIdx.x - beg[ii]);
     const uint32_t FB = d.NB / MSM_COARSE;
     const uint32_t* of = offs + space(b, SP_KEY)[ii] + (size_t)local * FB;   // local = j * 256 + h
@@ -499,26 +499,47 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     return p;
 }
 
-// plans measured by the autotuner on an MI355X (liblightzk_amd/tune/mi355x.json)
-static const hzk_tune_entry BUILTIN_PLANS[2][32] = {
+// plans measured by the autotuner on an MI355X (liblightzk_amd/tune/mi355x.json),
+// [regime][G1, G2][log2 n]
+static const hzk_tune_entry BUILTIN_PLANS[2][2][32] = {
 #include "tune_table.inc"
 };
 
-// plan for an MSM of n points: the context's tuned entry for the size class,
-// else the built-in measured plan, else the heuristic; a caller-forced window
-// size (or hzk_set_tuning) bypasses both tables
-MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c) {
+static int size_class(size_t n) {
+    int cls = 0;
+    while (((size_t)1 << (cls + 1)) <= n) ++cls;
+    return cls < 32 ? cls : 31;
+}
+
+static bool plan_from_entry(const hzk_tune_entry& t, size_t n, int force_g, size_t lds_bytes, MsmPlan* out) {
+    if (!t.c || ((size_t)4 << (t.c - 1)) > lds_bytes) return false;
+    MsmPlan q = msm_plan(n, t.c, t.g ? (int)t.g : force_g, lds_bytes);
+    if (t.K) q.K = t.K;
+    *out = q;
+    return true;
+}
+
+MsmPlan msm_plan_builtin(int g2, size_t n, int regime, size_t lds_bytes) {
+    MsmPlan p = msm_plan(n, 0, 0, lds_bytes);
+    const int cls = size_class(n);
+    if (regime == MSM_REGIME_BATCH && plan_from_entry(BUILTIN_PLANS[1][g2 ? 1 : 0][cls], n, 0, lds_bytes, &p)) return p;
+    plan_from_entry(BUILTIN_PLANS[0][g2 ? 1 : 0][cls], n, 0, lds_bytes, &p);
+    return p;
+}
+
+// plan for an MSM of n points: the context's tuned entry for the regime and
+// size class, else the built-in measured plan of the regime; a batch-regime
+// class with neither falls back to the single-MSM plan, and that one to the
+// heuristic. A caller-forced window size (or hzk_set_tuning) bypasses the
+// tables.
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c, int regime) {
     MsmPlan p = msm_plan(n, force_c ? force_c : ctx->force_window, ctx->force_seg, ctx->lds_per_block);
     if (!force_c && !ctx->force_window) {
-        int cls = 0;
-        while (((size_t)1 << (cls + 1)) <= n) ++cls;
-        cls = cls < 32 ? cls : 31;
-        hzk_tune_entry t = ctx->tune[g2 ? 1 : 0][cls];
-        if (!t.c && !ctx->force_seg) t = BUILTIN_PLANS[g2 ? 1 : 0][cls];
-        if (t.c && ((size_t)4 << (t.c - 1)) <= ctx->lds_per_block) {
-            MsmPlan q = msm_plan(n, t.c, t.g ? (int)t.g : ctx->force_seg, ctx->lds_per_block);
-            if (t.K) q.K = t.K;
-            p = q;
+        const int cls = size_class(n), gi = g2 ? 1 : 0;
+        for (int r = regime; r >= MSM_REGIME_SINGLE; --r) {
+            hzk_tune_entry t = ctx->tune[r][gi][cls];
+            if (!t.c && !ctx->force_seg) t = BUILTIN_PLANS[r][gi][cls];
+            if (plan_from_entry(t, n, ctx->force_seg, ctx->lds_per_block, &p)) break;
         }
     }
     return p;

@@ -24,7 +24,13 @@ struct MsmPlan {
 // Plan for n points: autotuned entry for the size class if one is loaded,
 // else the built-in heuristic; force_c / force_g override (0 = automatic).
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes);
-MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c);
+// plan regimes: an MSM with the GPU to itself, or an instance of a batch
+// whose digits total at least MSM_WIDE_BATCH_ENTRIES (two or more instances)
+constexpr int MSM_REGIME_SINGLE = 0, MSM_REGIME_BATCH = 1;
+constexpr uint64_t MSM_WIDE_BATCH_ENTRIES = (uint64_t)1 << 26;
+MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c, int regime = MSM_REGIME_SINGLE);
+// the built-in plan of a regime (no context overrides)
+MsmPlan msm_plan_builtin(int g2, size_t n, int regime, size_t lds_bytes);
 
 // Work spaces of the batched stages; each instance owns a contiguous range
 // [begin[s][i], begin[s][i+1]) of every space.
@@ -53,6 +59,7 @@ struct MsmInst {
     // pipeline) serves Lg consecutive digit windows, so W = ceil(Wd / Lg) and
     // n = nd * Lg virtual points. Plain bases: nd = n, Wd = W, Lg = 1.
     uint32_t nd, Wd, Lg;
+    uint32_t tp;           // scatter in two passes (coarse groups, then buckets)
 };
 
 struct MsmBatchDev {
@@ -77,7 +84,16 @@ constexpr uint32_t MSM_COARSE = 256;   // coarse key groups per window
 // Instances with many buckets per window and enough points that each coarse
 // group holds a few thousand entries scatter in two passes; below that the
 // per-group fixed cost outweighs the write coalescing.
-__host__ __device__ inline bool msm_two_pass(uint32_t n, uint32_t NB) { return NB >= 2048 && n >= (1u << 19); }
+// Two-pass scatter: alone, an instance pays for the extra pass from 2^19
+// points on; in a batch whose instances scatter together (batch_entries
+// digits in all), their random 4-byte writes share the L2 and the coalesced
+// two-pass form wins from MSM_TP_BATCH_MIN_N points on.
+constexpr uint64_t MSM_TP_BATCH_ENTRIES = (uint64_t)1 << 23;
+constexpr uint32_t MSM_TP_BATCH_MIN_N = 1u << 15;
+inline bool msm_two_pass(uint32_t n, uint32_t NB, uint64_t batch_entries) {
+    if (NB < 2048) return false;
+    return n >= (1u << 19) || (batch_entries >= MSM_TP_BATCH_ENTRIES && n >= MSM_TP_BATCH_MIN_N);
+}
 void msm_launch_scalar_stages(hzk_ctx* ctx, const MsmBatchDev& b, const uint64_t* host_totals, uint32_t max_nb,
                               const MsmScalarBufs& bufs);
 constexpr uint32_t MSM_SCAN_TILE = 1024;

@@ -16,6 +16,7 @@ namespace hzk {
 
 static constexpr int K2 = 16;        // fan-in of the tree over a hot bucket's chunk partials
 static constexpr uint32_t HOT = 16;  // a bucket spanning more chunks than this uses the tree
+static constexpr uint32_t HOT_LIST = 256;   // hot instances a hot-tree block lists in LDS
 
 HZK_DEV uint32_t find_inst_g(const uint64_t* __restrict__ beg, uint32_t ninst, uint64_t gid) {
     uint32_t lo = 0, hi = ninst;
@@ -107,38 +108,81 @@ __global__ void __launch_bounds__(256, 2) k_msm_accumulate(MsmBatchDev b, const 
 // over the head partials carry[2c] of chunks f+1..l (f, l: first and last
 // chunk of the bucket); afterwards carry[2(f+1)] holds their sum
 template <class F>
+HZK_DEV void hot_tree_chunk(const MsmBatchDev& b, const uint32_t* __restrict__ offs, const uint32_t* __restrict__ chunk2key,
+                            uint32_t stride, xyzz_t<F>* __restrict__ carry, uint32_t ii, uint32_t c) {
+    const MsmInst& d = b.inst[ii];
+    if ((uint64_t)c * d.K >= b.stats[4 * ii]) return;
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
+    const uint32_t* of = offs + space_g(b, SP_KEY)[ii];
+    const uint32_t k = chunk2key[cbeg[ii] + c], start = of[k];
+    if (start >= c * d.K) return;   // chunk c holds no head partial
+    const uint32_t f = start / d.K, l = (of[k + 1] - 1) / d.K;
+    if (l - f <= HOT) return;
+    const uint32_t rel = c - (f + 1), span = l - f;
+    if (rel % (stride * K2) != 0) return;
+    xyzz_t<F>* base = carry + 2 * cbeg[ii];
+    xyzz_t<F> acc = base[2 * c];
+    for (int i = 1; i < K2; ++i) {
+        uint32_t r = rel + i * stride;
+        if (r >= span) break;
+        acc.add(base[2 * (c + i * stride)]);
+    }
+    base[2 * c] = acc;
+}
+
+template <class F>
 __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint32_t* __restrict__ offs,
                                                       const uint32_t* __restrict__ chunk2key, uint32_t stride,
                                                       xyzz_t<F>* __restrict__ carry) {
-    // grid-stride over the chunk space: a small grid, since most instances
-    // have no hot bucket and their lanes leave at once
+    // Most instances have no hot bucket. Each block lists the instances that
+    // do (largest bucket > HOT chunks), in instance order, with the prefix
+    // sums of their chunk counts, all in LDS; the grid then strides over the
+    // concatenated chunks of the hot instances only, one (instance, chunk)
+    // per lane step.
+    __shared__ uint32_t hot[HOT_LIST];
+    __shared__ uint64_t hbeg[HOT_LIST + 1];
+    __shared__ uint64_t wmask[4];
+    __shared__ uint32_t nhot;
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
-    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < cbeg[b.ninst];
-         gid += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
-        const MsmInst& d = b.inst[ii];
-        if (b.stats[4 * ii + 2] <= HOT * d.K) {   // no bucket of this instance is hot: skip its chunks
-            const uint64_t S = (uint64_t)gridDim.x * blockDim.x, rem = cbeg[ii + 1] - gid;
-            gid += ((rem + S - 1) / S - 1) * S;   // the loop step then lands on the first chunk past ii
-            continue;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) nhot = 0;
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < b.ninst; i0 += 256) {
+        const uint32_t ii = i0 + threadIdx.x;
+        const bool h = ii < b.ninst && b.stats[4 * ii + 2] > HOT * b.inst[ii].K;
+        const uint64_t m = __ballot(h);
+        if (lane == 0) wmask[wave] = m;
+        __syncthreads();
+        uint32_t pos = nhot + __popcll(m & ((1ull << lane) - 1));
+        for (uint32_t w = 0; w < wave; ++w) pos += __popcll(wmask[w]);
+        if (h && pos < HOT_LIST) hot[pos] = ii;
+        __syncthreads();
+        if (threadIdx.x == 0) nhot += __popcll(wmask[0]) + __popcll(wmask[1]) + __popcll(wmask[2]) + __popcll(wmask[3]);
+        __syncthreads();
+    }
+    const uint32_t nh = nhot;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    if (nh > HOT_LIST) {   // more hot instances than the list holds: walk every chunk
+        for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < cbeg[b.ninst]; gid += step) {
+            const uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
+            if (b.stats[4 * ii + 2] > HOT * b.inst[ii].K)
+                hot_tree_chunk(b, offs, chunk2key, stride, carry, ii, (uint32_t)(gid - cbeg[ii]));
         }
-        const uint32_t c = (uint32_t)(gid - cbeg[ii]);
-        if ((uint64_t)c * d.K >= b.stats[4 * ii]) continue;
-        const uint32_t* of = offs + space_g(b, SP_KEY)[ii];
-        const uint32_t k = chunk2key[gid], start = of[k];
-        if (start >= c * d.K) continue;   // chunk c holds no head partial
-        const uint32_t f = start / d.K, l = (of[k + 1] - 1) / d.K;
-        if (l - f <= HOT) continue;
-        const uint32_t rel = c - (f + 1), span = l - f;
-        if (rel % (stride * K2) != 0) continue;
-        xyzz_t<F>* base = carry + 2 * cbeg[ii];
-        xyzz_t<F> acc = base[2 * c];
-        for (int i = 1; i < K2; ++i) {
-            uint32_t r = rel + i * stride;
-            if (r >= span) break;
-            acc.add(base[2 * (c + i * stride)]);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t h = 0; h < nh; ++h) { hbeg[h] = t; t += cbeg[hot[h] + 1] - cbeg[hot[h]]; }
+        hbeg[nh] = t;
+    }
+    __syncthreads();
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < hbeg[nh]; g += step) {
+        uint32_t lo = 0, hi = nh;   // hbeg[lo] <= g < hbeg[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (hbeg[mid] <= g) lo = mid; else hi = mid;
         }
-        base[2 * c] = acc;
+        hot_tree_chunk(b, offs, chunk2key, stride, carry, hot[lo], (uint32_t)(g - hbeg[lo]));
     }
 }
 
@@ -241,6 +285,7 @@ struct MsmBatchLayout {
     uint32_t max_nb = 1;
     uint32_t tree_levels = 0;      // tree levels for hot buckets (upper bound)
     bool two_pass = false;         // some instance scatters in two passes
+    int regime = 0;                // plan regime (MSM_REGIME_*)
     size_t bytes = 0;              // scratch needed
     // scratch offsets
     size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_coarse, o_chunk2key, o_bucket,
@@ -252,8 +297,22 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     MsmBatchLayout L;
     L.inst.resize(count);
     L.begin.assign((size_t)SP_COUNT * (count + 1), 0);
+    const int g2 = sizeof(F) == 32 ? 0 : 1;
+    // regime: a batch of two or more instances whose digits (under the
+    // single-MSM plans) total MSM_WIDE_BATCH_ENTRIES fills the GPU by itself,
+    // so its instances take the throughput plans (fewer buckets, longer runs)
+    int regime = ctx->force_regime;
+    if (regime < 0) {
+        uint64_t e = 0;
+        for (size_t i = 0; i < count; ++i) {
+            const MsmPlan q = msm_plan_ctx(ctx, g2, jobs[i].n, (int)jobs[i].win_siz, MSM_REGIME_SINGLE);
+            e += (uint64_t)jobs[i].n * (uint64_t)q.W;
+        }
+        regime = count >= 2 && e >= MSM_WIDE_BATCH_ENTRIES ? MSM_REGIME_BATCH : MSM_REGIME_SINGLE;
+    }
+    L.regime = regime;
     for (size_t i = 0; i < count; ++i) {
-        MsmPlan p = msm_plan_ctx(ctx, sizeof(F) == 32 ? 0 : 1, jobs[i].n, (int)jobs[i].win_siz);
+        MsmPlan p = msm_plan_ctx(ctx, g2, jobs[i].n, (int)jobs[i].win_siz, regime);
         MsmInst& d = L.inst[i];
         d.scalars = jobs[i].scalars;
         d.bases = jobs[i].bases;
@@ -299,8 +358,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         sz[SP_WIN] = (uint64_t)d.W * d.S;
         sz[SP_ENTRY] = (entries + 63) & ~(uint64_t)63;   // 256-B aligned per instance (vector index loads)
         sz[SP_HIST] = (uint64_t)d.T * d.nk;
-        sz[SP_CGROUP] = msm_two_pass(d.n, d.NB) ? (uint64_t)d.W * MSM_COARSE : 0;
-        if (msm_two_pass(d.n, d.NB)) L.two_pass = true;
+        d.tp = 0;   // two-pass flag and coarse groups: set below, once the batch's digit total is known
+        sz[SP_CGROUP] = 0;
         for (int s = 0; s < SP_COUNT; ++s) L.begin[(size_t)s * (count + 1) + i + 1] = L.begin[(size_t)s * (count + 1) + i] + sz[s];
         if (d.NB > L.max_nb) L.max_nb = d.NB;
         uint64_t maxspan = (d.n + d.K - 1) / d.K + 1;   // a bucket holds at most n entries
@@ -308,6 +367,16 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         if (maxspan > HOT)
             for (uint64_t stride = 1; stride < maxspan; stride *= K2) ++lv;
         if (lv > L.tree_levels) L.tree_levels = lv;
+    }
+    // two-pass scatter per instance, then the coarse-group space it needs
+    uint64_t batch_entries = 0;
+    for (const MsmInst& d : L.inst) batch_entries += (uint64_t)d.n * d.W;
+    for (size_t i = 0; i < count; ++i) {
+        MsmInst& d = L.inst[i];
+        d.tp = msm_two_pass(d.n, d.NB, batch_entries) ? 1u : 0u;
+        if (d.tp) L.two_pass = true;
+        const size_t s = SP_CGROUP;
+        L.begin[s * (count + 1) + i + 1] = L.begin[s * (count + 1) + i] + (d.tp ? (uint64_t)d.W * MSM_COARSE : 0);
     }
     for (int s = 0; s < SP_COUNT; ++s) L.tot[s] = L.begin[(size_t)s * (count + 1) + count];
     size_t off = 0;

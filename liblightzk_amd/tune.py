@@ -3,19 +3,25 @@ and size class, produced by the library's autotuner (hzk_msm_autotune) on a
 given GPU and loaded into a context before use.
 
 File format (JSON): {"device": str, "g1": {"<log_n>": [c, K, g], ...},
-"g2": {...}}."""
+"g2": {...}, "g1_batch": {...}, "g2_batch": {...}}; the *_batch tables hold
+the plans of the wide-batch (throughput) regime."""
 import json
 import os
 
 DEFAULT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune", "mi355x.json")
 
 
-def autotune(ctx, classes_g1=(10, 24), classes_g2=(10, 22), budget_ms=4000.0):
-    out = {"device": "gfx950", "g1": {}, "g2": {}}
+def autotune(ctx, classes_g1=(10, 24), classes_g2=(10, 22), budget_ms=4000.0, batch=False):
+    """Tune the single-MSM regime (batch=False) or the wide-batch regime."""
+    out = {"device": "gfx950"}
     for group, (lo, hi), key in ((1, classes_g1, "g1"), (2, classes_g2, "g2")):
         if lo is None:
             continue
-        res = ctx.msm_autotune(group, lo, hi, budget_ms)
+        if batch:
+            res = ctx.msm_batch_autotune(group, lo, hi, budget_ms)
+            key += "_batch"
+        else:
+            res = ctx.msm_autotune(group, lo, hi, budget_ms)
         out[key] = {str(lg): list(v) for lg, v in res.items()}
     return out
 
@@ -36,5 +42,8 @@ def load(ctx, path=DEFAULT):
     for group, key in ((1, "g1"), (2, "g2")):
         for lg, (c, K, g) in table.get(key, {}).items():
             ctx.msm_tune_set(group, int(lg), int(c), int(K), int(g))
+            n += 1
+        for lg, (c, K, g) in table.get(key + "_batch", {}).items():
+            ctx.msm_batch_tune_set(group, int(lg), int(c), int(K), int(g))
             n += 1
     return n

@@ -46,3 +46,20 @@ def test_product_generator_matches_oracle(orc):
         b = orc.gen_fr(5, 3, 5000, sparse=okind)
         assert np.array_equal(a, b), kind
     assert np.array_equal(liblightzk_amd.gen_field(6, 1, 999, "fq"), orc.gen_fq(6, 1, 999))
+
+
+def test_builtin_plan_table_matches_json(tmp_path):
+    """The compiled-in MSM plan table (tune_table.inc, both regimes) is the
+    one gen_table.py produces from the shipped mi355x.json."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    here = os.path.join(root, "liblightzk_amd", "tune")
+    out = tmp_path / "t.inc"
+    subprocess.run([sys.executable, os.path.join(here, "gen_table.py"), os.path.join(here, "mi355x.json"), str(out)],
+                   check=True)
+    shipped = open(os.path.join(root, "liblightzk_amd", "csrc", "tune_table.inc")).read()
+    assert out.read_text() == shipped
+    # two regimes x two groups x 32 size classes
+    rows = [ln for ln in shipped.splitlines() if ln.startswith("{{")]
+    assert len(rows) == 4 and all(r.count("u}") == 32 for r in rows)

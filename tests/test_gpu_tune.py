@@ -110,3 +110,87 @@ def test_level_tables_single_bucket_set(ctx, orc, data, c):
             assert np.array_equal(outs[1].download(np.uint64, width), orc.msm_closed_form(s, k, group))
             assert np.array_equal(outs[2].download(np.uint64, width),
                                   orc.msm_closed_form(s[: 4 * 1000].copy(), k[: 4 * 1000].copy(), group))
+
+
+BATCH_PLANS = [(6, 16, 4), (9, 64, 32), (11, 32, 64), (13, 128, 16)]
+
+
+@pytest.mark.parametrize("plan", BATCH_PLANS)
+def test_batch_regime_plans_exact(ctx, orc, data, plan):
+    """Wide-batch plans (forced regime) give the closed form, in a batch and
+    alone, for G1 and G2."""
+    n, s, sp, k, b1, b2 = data
+    c, K, g = plan
+    lg = n.bit_length() - 1
+    ctx.set_msm_regime(1)
+    try:
+        for group, bases, width in ((1, b1, 12), (2, b2, 24)):
+            ctx.msm_batch_tune_set(group, lg, c, K, g)
+            assert ctx.msm_batch_tune_get(group, lg) == (c, K, g)
+            db = ctx.upload(bases)
+            for sc in (s, sp):
+                ds = ctx.upload(sc)
+                outs = [ctx.alloc(8 * width) for _ in range(3)]
+                ctx.msm_batch(group, [(ds, db, n, outs[0]), (ds, db, 1000, outs[1]), (ds, db, n - 1, outs[2])])
+                assert np.array_equal(outs[0].download(np.uint64, width), orc.msm_closed_form(sc, k, group)), plan
+                assert np.array_equal(outs[1].download(np.uint64, width),
+                                      orc.msm_closed_form(sc[: 4 * 1000].copy(), k[: 4 * 1000].copy(), group))
+                assert np.array_equal(outs[2].download(np.uint64, width),
+                                      orc.msm_closed_form(sc[: 4 * (n - 1)].copy(), k[: 4 * (n - 1)].copy(), group))
+                d = ctx.alloc(8 * width)
+                (ctx.msm_g1 if group == 1 else ctx.msm_g2)(ds, db, n, d)
+                assert np.array_equal(d.download(np.uint64, width), orc.msm_closed_form(sc, k, group)), plan
+    finally:
+        ctx.set_msm_regime(-1)
+        ctx.msm_batch_tune_set(1, lg, 0)
+        ctx.msm_batch_tune_set(2, lg, 0)
+
+
+def test_wide_batch_takes_batch_plans_same_results(orc):
+    """Two 2^21-point G1 MSMs (2^26 digits under the single plans) run as a
+    batch take the class's wide-batch plan, and give the same points as each
+    MSM run alone under the single-MSM plan."""
+    import liblightzk_amd as z
+    n = 1 << 21
+    with z.Context(0) as c2:
+        k = z.gen_field(9, 1, n)
+        s = z.gen_field(9, 0, n + 64)
+        dk, ds = c2.upload(k), c2.upload(s)
+        db = c2.alloc(64 * n)
+        c2.fixbase_g1(dk, n, db)
+        c2.msm_batch_tune_set(1, 21, 14, 64, 32)
+        single_c = c2.msm_tune_get(1, 21)[0]
+        assert single_c != 14
+        outs = [c2.alloc(96), c2.alloc(96)]
+        c2.msm_batch(1, [(ds, db, n, outs[0]), (ds.offset(32 * 64), db, n, outs[1])])
+        assert c2.msm_last_stats()["c"] == 14
+        batch = [o.download(np.uint64, 12) for o in outs]
+        for i, off in enumerate((0, 32 * 64)):
+            d = c2.alloc(96)
+            c2.msm_g1(ds.offset(off), db, n, d)
+            assert c2.msm_last_stats()["c"] == single_c
+            assert np.array_equal(d.download(np.uint64, 12), batch[i])
+        # and one of them against the closed form
+        assert np.array_equal(batch[0], orc.msm_closed_form(s[: 4 * n].copy(), k, 1))
+
+
+def test_batch_autotune_small_classes(orc, data, tmp_path):
+    import liblightzk_amd as z
+    from liblightzk_amd import tune
+    n, s, sp, k, b1, b2 = data
+    with z.Context(0) as c2:
+        res = c2.msm_batch_autotune(1, 8, 10, budget_ms=3000)
+        for lg, (c, K, g) in res.items():
+            assert 4 <= c <= 16 and K % 4 == 0 and g & (g - 1) == 0 and g <= 1 << (c - 1), (lg, c, K, g)
+        c2.set_msm_regime(1)
+        ds, db = c2.upload(s), c2.upload(b1)
+        outs = [c2.alloc(96) for _ in range(3)]
+        c2.msm_batch(1, [(ds, db, 300, outs[0]), (ds, db, 700, outs[1]), (ds, db, 1500, outs[2])])
+        for m, o in zip((300, 700, 1500), outs):
+            assert np.array_equal(o.download(np.uint64, 12), orc.msm_closed_form(s[: 4 * m].copy(), k[: 4 * m].copy(), 1))
+        path = str(tmp_path / "plans.json")
+        tune.save({"device": "gfx950", "g1_batch": {str(lg): list(v) for lg, v in res.items()}}, path)
+    with z.Context(0) as c3:
+        assert tune.load(c3, path) == len(res)
+        for lg, v in res.items():
+            assert c3.msm_batch_tune_get(1, lg) == tuple(v)
