@@ -344,7 +344,7 @@ static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
         const size_t coarse = p.NB >= 2048 && n >= MSM_TP_BATCH_MIN_N ? entries * 8 : 0;   // two-pass staging
         size_t b = entries * 8 + coarse + (size_t)p.T * nk * 4 + 2 * (nk + 1) * 4 +
                    2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 + (nk + 1) * pt_bytes + chunks * (4 + 2 * pt_bytes) +
-                   (bred + W) * pt_bytes + 4096;
+                   2 * (bred + 64 * W) * pt_bytes + 4096;   // run (A, R) and part (P, RS) pairs
         worst = std::max(worst, b);
     }
     return worst;

@@ -14,7 +14,10 @@
 
 namespace hzk {
 
-static constexpr int K2 = 16;        // fan-in of the tree over a hot bucket's chunk partials
+// fan-in of the tree over a hot bucket's chunk partials: each level is a
+// chain of K2 - 1 dependent additions on few lanes, so latency, not work,
+// sets its cost (4: three additions per level)
+static constexpr int K2 = 4;
 static constexpr uint32_t HOT = 16;  // a bucket spanning more chunks than this uses the tree
 static constexpr uint32_t HOT_LIST = 256;   // hot instances a hot-tree block lists in LDS
 
@@ -216,7 +219,16 @@ __global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint3
 }
 
 // ------------------------------------------------------------------ 7. bucket reduction
-// lane per (window, run of g buckets): T = sum_{d in run} d * B_d
+// A window's value is sum_{d=1..NB} d * B_d. Bucket d = r*g + q + 1 lies in
+// run r (g buckets) at offset q, so the sum is
+//     sum_r [ A_r + g * r * R_r ],   A_r = sum_q (q+1) B,   R_r = sum_q B.
+// Stage 7a: lane per (window, run): A_r and R_r by running sums (2g
+// additions, no scalar multiplication). Stage 7b: workgroup per (window,
+// part of Gs = G/S runs): thread t takes m = Gs/256 consecutive runs; the
+// run weights r become t*m + i inside the part, and sum_t t * rr_t (rr_t the
+// thread's run sum) is the sum over t >= 1 of the suffix sums of rr, one
+// LDS scan. Each part yields its value P_p (weights relative to the part's
+// first run) and its run sum RS_p; the host adds g * p * Gs * RS_p.
 template <class F>
 __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xyzz_t<F>* __restrict__ bucket,
                                                          const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ segsum) {
@@ -237,44 +249,90 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xy
         if (of[key + 1] != of[key]) run.add(bk[key]);
         acc.add(run);
     }
-    // acc = sum (d - sg*g) B_d ; add (sg*g) * run
-    uint32_t w = sg * d.g;
-    if (w) {
-        xyzz_t<F> m = xyzz_t<F>::inf();
-        for (int bit = 31 - __clz(w); bit >= 0; --bit) {
-            m = m.dbl();
-            if ((w >> bit) & 1) m.add(run);
-        }
-        acc.add(m);
-    }
-    segsum[gid] = acc;
+    segsum[2 * gid] = acc;       // A_r
+    segsum[2 * gid + 1] = run;   // R_r
 }
 
-// S workgroups per window, each a tree sum of G/S of the window's run sums;
-// the host adds the S partials of a window before the combine
-template <class F>
-__global__ void __launch_bounds__(256) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
-                                                    xyzz_t<F>* __restrict__ winsum) {
-    __shared__ xyzz_t<F> sh[256];
+// tree sum of sh[0..T) into sh[0]
+template <class F, int T>
+HZK_DEV void lds_tree_sum(xyzz_t<F>* sh, uint32_t t) {
+    for (uint32_t off = T / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            xyzz_t<F> x = sh[t];
+            x.add(sh[t + off]);
+            sh[t] = x;
+        }
+        __syncthreads();
+    }
+}
+
+// T threads per part: 256 for single MSMs (shallow: the kernel is latency
+// bound there), 64 in wide batches (each thread takes 4x more runs, so the
+// scan and tree overhead per run is 4x smaller)
+template <class F, int T>
+__global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
+                                                  xyzz_t<F>* __restrict__ winsum) {
+    __shared__ xyzz_t<F> sh[T];
     const uint64_t* wbeg = space_g(b, SP_WIN);
     uint32_t ii = find_inst_g(wbeg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
     const uint32_t local = (uint32_t)(blockIdx.x - wbeg[ii]);
     const uint32_t j = local / d.S, part = local % d.S, Gs = d.G / d.S;
-    const xyzz_t<F>* ss = segsum + space_g(b, SP_BRED)[ii] + (size_t)j * d.G + (size_t)part * Gs;
-    xyzz_t<F> acc = xyzz_t<F>::inf();
-    for (uint32_t s = threadIdx.x; s < Gs; s += 256) acc.add(ss[s]);
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) {
-            xyzz_t<F> a = sh[threadIdx.x];
-            a.add(sh[threadIdx.x + off]);
-            sh[threadIdx.x] = a;
+    const uint32_t m = Gs >= T ? Gs / T : 1, t = threadIdx.x;
+    // runs t*m .. t*m + m - 1 of the part (A_r at 2r, R_r at 2r + 1)
+    const xyzz_t<F>* ss = segsum + 2 * (space_g(b, SP_BRED)[ii] + (size_t)j * d.G + (size_t)part * Gs + (size_t)t * m);
+    xyzz_t<F> a = xyzz_t<F>::inf(), run = xyzz_t<F>::inf(), w = xyzz_t<F>::inf();
+    if (t * m < Gs)
+        for (int i = (int)m - 1; i >= 0; --i) {
+            a.add(ss[2 * i]);
+            run.add(ss[2 * i + 1]);
+            if (i > 0) w.add(run);   // w = sum_i i * R_i
         }
+    // inclusive suffix scan of the thread run sums: sh[t] = sum_{t' >= t} rr_t'
+    sh[t] = run;
+    __syncthreads();
+    for (uint32_t off = 1; off < T; off <<= 1) {
+        xyzz_t<F> x = sh[t];
+        if (t + off < T) x.add(sh[t + off]);
+        __syncthreads();
+        sh[t] = x;
         __syncthreads();
     }
-    if (threadIdx.x == 0) winsum[blockIdx.x] = sh[0];
+    const xyzz_t<F> part_sum = sh[0];
+    // P = sum_t a_t + g * sum_t (w_t + m * X_t), X_t = sh[t] for t >= 1
+    xyzz_t<F> u = t ? sh[t] : xyzz_t<F>::inf();
+    for (uint32_t k = 1; k < m; k <<= 1) u = u.dbl();
+    u.add(w);
+    if (T == 256) {
+        // shallow: the factor g per thread, one tree
+        for (uint32_t k = 1; k < d.g; k <<= 1) u = u.dbl();
+        u.add(a);
+        __syncthreads();
+        sh[t] = u;
+        __syncthreads();
+        lds_tree_sum<F, T>(sh, t);
+        if (t == 0) {
+            winsum[2 * blockIdx.x] = sh[0];         // P_p
+            winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
+        }
+    } else {
+        // less work: both sums first, the factor g once
+        __syncthreads();
+        sh[t] = u;
+        __syncthreads();
+        lds_tree_sum<F, T>(sh, t);
+        xyzz_t<F> total = sh[0];
+        __syncthreads();
+        sh[t] = a;
+        __syncthreads();
+        lds_tree_sum<F, T>(sh, t);
+        if (t == 0) {
+            for (uint32_t k = 1; k < d.g; k <<= 1) total = total.dbl();
+            total.add(sh[0]);
+            winsum[2 * blockIdx.x] = total;         // P_p
+            winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
+        }
+    }
 }
 
 // ------------------------------------------------------------------ driver
@@ -394,8 +452,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_chunk2key = take(L.tot[SP_CHUNK] * 4);
     L.o_bucket = take(L.tot[SP_KEY] * sizeof(xyzz_t<F>));
     L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
-    L.o_segsum = take(L.tot[SP_BRED] * sizeof(xyzz_t<F>));
-    L.o_win = take(L.tot[SP_WIN] * sizeof(xyzz_t<F>));
+    L.o_segsum = take(2 * L.tot[SP_BRED] * sizeof(xyzz_t<F>));   // A_r, R_r per run
+    L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));       // P_p, RS_p per part
     L.bytes = off + 256;
     return L;
 }
@@ -445,13 +503,17 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
         TimedLaunch tl(ctx, "msm_bucket_reduce");
         hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b,
                            (const xyzz_t<F>*)bucket, (const uint32_t*)sb.offs, segsum);
-        hipLaunchKernelGGL(k_window_sum<F>, dim3((unsigned)L.tot[SP_WIN]), dim3(256), 0, st, b, (const xyzz_t<F>*)segsum, win);
+        if (L.regime == MSM_REGIME_BATCH)
+            hipLaunchKernelGGL((k_window_sum<F, 64>), dim3((unsigned)L.tot[SP_WIN]), dim3(64), 0, st, b, (const xyzz_t<F>*)segsum, win);
+        else
+            hipLaunchKernelGGL((k_window_sum<F, 256>), dim3((unsigned)L.tot[SP_WIN]), dim3(256), 0, st, b, (const xyzz_t<F>*)segsum, win);
     }
 }
 
 template <class F>
 static std::string shape_key(const MsmBatchLayout& L) {
     std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
+    k += L.regime == MSM_REGIME_BATCH ? "b:" : "s:";
     for (const MsmInst& d : L.inst) {
         char buf[64];
         snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T, d.Lg);
@@ -483,7 +545,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
               ctx->lds_per_block);
     int rc = ensure_scratch(ctx, L.bytes);
     if (rc) return rc;
-    const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 + 1024;
+    const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 + 1024;
     char* pin;
     int slot;
     rc = stage_acquire(ctx, host_bytes, &pin, &slot);
@@ -521,15 +583,41 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     }
     HZK_HIP(ctx, hipGetLastError());
     char* hwin = pin + count * sizeof(MsmInst) + L.begin.size() * 8;
-    uint32_t* hstats = (uint32_t*)(hwin + L.tot[SP_WIN] * sizeof(xyzz_t<F>));
-    HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
+    uint32_t* hstats = (uint32_t*)(hwin + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));
+    HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
     HZK_HIP(ctx, hipMemcpyAsync(hstats, base + L.o_stats, count * 16, hipMemcpyDeviceToHost, st));
     rc = stage_fence(ctx, slot);
     if (rc) return rc;
     HZK_HIP(ctx, hipStreamSynchronize(st));
 
-    // window combine (Horner over windows) on the host, instances in parallel
+    // window values on the host, all (instance, window) pairs in parallel:
+    // T_j = sum_p P_p + (g Gs) * sum_p p RS_p, the last sum as the sum of the
+    // suffix sums of RS over p >= 1
     const uint64_t* wbeg = L.begin.data() + (size_t)SP_WIN * (count + 1);
+    std::vector<uint64_t> tbeg(count + 1, 0);
+    for (size_t i = 0; i < count; ++i) tbeg[i + 1] = tbeg[i] + L.inst[i].W;
+    std::vector<HostPt> T(tbeg[count]);
+#pragma omp parallel for schedule(dynamic, 1) if (tbeg[count] > 1)
+    for (long q = 0; q < (long)tbeg[count]; ++q) {
+        const size_t i = std::upper_bound(tbeg.begin(), tbeg.end(), (uint64_t)q) - tbeg.begin() - 1;
+        const MsmInst& d = L.inst[i];
+        const size_t j = (size_t)q - tbeg[i];
+        auto pt = [&](uint32_t p, int which) {
+            return HostPt::load_xyzz(hwin + (2 * (wbeg[i] + j * d.S + p) + which) * sizeof(xyzz_t<F>));
+        };
+        HostPt acc = HostPt::inf(), suf = HostPt::inf(), y = HostPt::inf();
+        for (uint32_t p = 0; p < d.S; ++p) acc = acc.add(pt(p, 0));
+        for (uint32_t p = d.S - 1; p >= 1; --p) {
+            suf = suf.add(pt(p, 1));
+            y = y.add(suf);
+        }
+        if (d.S > 1) {
+            for (uint32_t k = 1; k < d.G / d.S * d.g; k <<= 1) y = y.dbl();
+            acc = acc.add(y);
+        }
+        T[q] = acc;
+    }
+    // Horner over windows, instances in parallel
     std::vector<char> out(count * 3 * sizeof(F));
 #pragma omp parallel for schedule(dynamic, 1) if (count > 1)
     for (long i = 0; i < (long)count; ++i) {
@@ -537,8 +625,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HostPt acc = HostPt::inf();
         for (int j = (int)d.W - 1; j >= 0; --j) {
             for (uint32_t q = 0; q < d.c * d.Lg; ++q) acc = acc.dbl();   // a bucket set spans Lg digit windows
-            for (uint32_t p = 0; p < d.S; ++p)
-                acc = acc.add(HostPt::load_xyzz(hwin + (wbeg[i] + (size_t)j * d.S + p) * sizeof(xyzz_t<F>)));
+            acc = acc.add(T[tbeg[i] + j]);
         }
         acc.store_jacobian(&out[i * 3 * sizeof(F)]);
     }
