@@ -93,6 +93,63 @@ HZK_DEV void mac3_vs(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t x1, uin
         : "v"(x0), "v"(x1), "v"(x2), "s"(ys));
 }
 
+// The same steps for the first product of a column: the carry word is
+// written (top = carry-out) instead of accumulated, so the previous column's
+// carry word never has to be cleared.
+HZK_DEV void mac2_vv_first(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %8, %9, %1\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %2, %4, 0, 0, %4\n\t"
+        "v_addc_co_u32_e64 %3, %5, 0, 0, %5"
+        : "+v"(acc[0]), "+v"(acc[1]), "=&v"(top[0]), "=&v"(top[1]), "=&s"(c0), "=&s"(c1)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
+}
+HZK_DEV void mac2_vs_first(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t x1, uint32_t ys) {
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %8, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %7, %8, %1\n\t"
+        "s_nop 0\n\t"
+        "v_addc_co_u32_e64 %2, %4, 0, 0, %4\n\t"
+        "v_addc_co_u32_e64 %3, %5, 0, 0, %5"
+        : "+v"(acc[0]), "+v"(acc[1]), "=&v"(top[0]), "=&v"(top[1]), "=&s"(c0), "=&s"(c1)
+        : "v"(x0), "v"(x1), "s"(ys));
+}
+HZK_DEV void mac3_vv_first(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t x2,
+                           uint32_t y2) {
+    uint64_t c0, c1, c2;
+    asm("v_mad_u64_u32 %0, %6, %9, %10, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %11, %12, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %13, %14, %2\n\t"
+        "v_addc_co_u32_e64 %3, %6, 0, 0, %6\n\t"
+        "v_addc_co_u32_e64 %4, %7, 0, 0, %7\n\t"
+        "v_addc_co_u32_e64 %5, %8, 0, 0, %8"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "=&v"(top[0]), "=&v"(top[1]), "=&v"(top[2]), "=&s"(c0), "=&s"(c1),
+          "=&s"(c2)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2));
+}
+HZK_DEV void mac3_vs_first(uint64_t* acc, uint32_t* top, uint32_t x0, uint32_t x1, uint32_t x2, uint32_t ys) {
+    uint64_t c0, c1, c2;
+    asm("v_mad_u64_u32 %0, %6, %9, %12, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %10, %12, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %11, %12, %2\n\t"
+        "v_addc_co_u32_e64 %3, %6, 0, 0, %6\n\t"
+        "v_addc_co_u32_e64 %4, %7, 0, 0, %7\n\t"
+        "v_addc_co_u32_e64 %5, %8, 0, 0, %8"
+        : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "=&v"(top[0]), "=&v"(top[1]), "=&v"(top[2]), "=&s"(c0), "=&s"(c1),
+          "=&s"(c2)
+        : "v"(x0), "v"(x1), "v"(x2), "s"(ys));
+}
+
+// x * y mod 2^32 as the low half of a v_mad_u64_u32 (the reduction digits
+// m_k); measured slightly faster in the MSM loops than v_mul_lo_u32
+HZK_DEV uint32_t mul_lo_mad(uint32_t x, uint32_t ys) {
+    uint64_t r, c;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, 0" : "=v"(r), "=s"(c) : "v"(x), "s"(ys));
+    return (uint32_t)r;
+}
+
 template <class P>
 struct fp {
     uint32_t l[8];
@@ -216,38 +273,49 @@ struct fp {
         static_assert(N == 2 || N == 3, "2 or 3 chains");
         uint32_t m[N][8], t[N][8];
         uint64_t acc[N];
-        uint32_t top[N];
+        uint32_t top[N];   // written by each column's first product
 #pragma unroll
-        for (int c = 0; c < N; ++c) { acc[c] = 0; top[c] = 0; }
-        auto vv = [&](int i, int j) {
-            if constexpr (N == 2) mac2_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j]);
-            else mac3_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j], a[2][i], b[2][j]);
+        for (int c = 0; c < N; ++c) acc[c] = 0;
+        // first: the column's first product (its carry word starts from the carry-out)
+        auto vv = [&](int i, int j, bool first) {
+            if constexpr (N == 2) {
+                if (first) mac2_vv_first(acc, top, a[0][i], b[0][j], a[1][i], b[1][j]);
+                else mac2_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j]);
+            } else {
+                if (first) mac3_vv_first(acc, top, a[0][i], b[0][j], a[1][i], b[1][j], a[2][i], b[2][j]);
+                else mac3_vv(acc, top, a[0][i], b[0][j], a[1][i], b[1][j], a[2][i], b[2][j]);
+            }
         };
-        auto vs = [&](int j, uint32_t ys) {
-            if constexpr (N == 2) mac2_vs(acc, top, m[0][j], m[1][j], ys);
-            else mac3_vs(acc, top, m[0][j], m[1][j], m[2][j], ys);
+        auto vs = [&](int j, uint32_t ys, bool first) {
+            if constexpr (N == 2) {
+                if (first) mac2_vs_first(acc, top, m[0][j], m[1][j], ys);
+                else mac2_vs(acc, top, m[0][j], m[1][j], ys);
+            } else {
+                if (first) mac3_vs_first(acc, top, m[0][j], m[1][j], m[2][j], ys);
+                else mac3_vs(acc, top, m[0][j], m[1][j], m[2][j], ys);
+            }
         };
         auto shift = [&]() {
 #pragma unroll
-            for (int c = 0; c < N; ++c) { acc[c] = (acc[c] >> 32) | ((uint64_t)top[c] << 32); top[c] = 0; }
+            for (int c = 0; c < N; ++c) acc[c] = (acc[c] >> 32) | ((uint64_t)top[c] << 32);
         };
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
 #pragma unroll
-            for (int j = 0; j < k; ++j) vs(j, P::MOD[k - j]);
+            for (int j = 0; j < k; ++j) vs(j, P::MOD[k - j], j == 0);
 #pragma unroll
-            for (int j = 0; j <= k; ++j) vv(j, k - j);
+            for (int j = 0; j <= k; ++j) vv(j, k - j, k == 0 && j == 0);
 #pragma unroll
-            for (int c = 0; c < N; ++c) m[c][k] = (uint32_t)acc[c] * P::M0;
-            vs(k, P::MOD[0]);
+            for (int c = 0; c < N; ++c) m[c][k] = mul_lo_mad((uint32_t)acc[c], P::M0);
+            vs(k, P::MOD[0], false);
             shift();
         }
 #pragma unroll
         for (int k = 8; k < 15; ++k) {
 #pragma unroll
-            for (int j = k - 7; j < 8; ++j) vv(j, k - j);
+            for (int j = k - 7; j < 8; ++j) vv(j, k - j, j == k - 7);
 #pragma unroll
-            for (int j = k - 7; j < 8; ++j) vs(j, P::MOD[k - j]);
+            for (int j = k - 7; j < 8; ++j) vs(j, P::MOD[k - j], false);
 #pragma unroll
             for (int c = 0; c < N; ++c) t[c][k - 8] = (uint32_t)acc[c];
             shift();
