@@ -47,7 +47,9 @@ double now_s() {
 struct Request {
     uint64_t id;
     int circuit;
-    std::vector<uint8_t> z;
+    std::vector<uint8_t> z;   // witness, pageable (when no pinned buffer was free)
+    void* zp = nullptr;       // witness in a pooled pinned buffer
+    size_t zbytes = 0;
     uint8_t rs[64];
     uint8_t proof[256];
     int status = 0;   // 0 pending, 1 done, < 0 failed
@@ -81,6 +83,11 @@ struct hzk_server {
     // synchronise the device and stall the other stage)
     std::vector<std::vector<BufSet>> free_bufs;
     size_t cached_bytes = 0;
+    // pinned host buffers for witnesses, pooled per circuit: a copy from
+    // pageable memory is staged synchronously by the runtime and would hold
+    // the worker thread (and the GPU behind it) for the whole transfer
+    std::vector<std::vector<void*>> pinned_free;
+    size_t pinned_bytes = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
@@ -150,6 +157,52 @@ int alloc_request(hzk_server* s, Request* r) {
     return 0;
 }
 
+constexpr size_t PINNED_CAP = (size_t)2 << 30;   // pinned witness bytes at most
+
+// witness of r into a pinned buffer of its circuit's pool (a new one while
+// the pool is under PINNED_CAP), else into pageable memory
+void stage_witness(hzk_server* s, Request* r, const void* src) {
+    void* p = nullptr;
+    bool grow = false;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto& fl = s->pinned_free[r->circuit];
+        if (!fl.empty()) {
+            p = fl.back();
+            fl.pop_back();
+        } else if (s->pinned_bytes + r->zbytes <= PINNED_CAP) {
+            s->pinned_bytes += r->zbytes;
+            grow = true;
+        }
+    }
+    if (grow) {
+        (void)hipSetDevice(s->device);
+        if (hipHostMalloc(&p, r->zbytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->pinned_bytes -= r->zbytes;
+        }
+    }
+    if (p) {
+        std::memcpy(p, src, r->zbytes);
+        r->zp = p;
+    } else {
+        r->z.assign((const uint8_t*)src, (const uint8_t*)src + r->zbytes);
+    }
+}
+
+// the pinned witness buffer back to its pool; only once its upload is done
+void unstage_witness_locked(hzk_server* s, Request* r) {
+    if (!r->zp) return;
+    s->pinned_free[r->circuit].push_back(r->zp);
+    r->zp = nullptr;
+}
+void unstage_witness(hzk_server* s, std::vector<Request*>& rs) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Request* r : rs) unstage_witness_locked(s, r);
+}
+
 // back to the circuit's free list (caller holds no lock)
 void free_request(hzk_server* s, Request* r) {
     if (!r->dz) return;
@@ -184,7 +237,8 @@ int stage_h(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     if (rs.empty()) return 0;
     std::vector<G16Req> reqs;
     for (Request* r : rs) {
-        HZK_HIP(ctx, hipMemcpyAsync(r->dz, r->z.data(), r->z.size(), hipMemcpyHostToDevice, ctx->stream));
+        const void* src = r->zp ? r->zp : (const void*)r->z.data();
+        HZK_HIP(ctx, hipMemcpyAsync(r->dz, src, r->zbytes, hipMemcpyHostToDevice, ctx->stream));
         reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
     }
     return g16_h_batch(ctx, reqs.data(), reqs.size());
@@ -220,6 +274,7 @@ void worker_loop(hzk_server* s, Pipeline* p) {
                 Request* r = s->pending.front();
                 if (r->footprint > s->budget) {   // can never fit: fail it
                     s->pending.pop_front();
+                    unstage_witness_locked(s, r);
                     r->status = HZK_ERR_NOMEM;
                     r->t_done = now_s();
                     ++s->failed;
@@ -244,6 +299,7 @@ void worker_loop(hzk_server* s, Pipeline* p) {
         for (Request* r : admitted) {
             if (alloc_request(s, r)) {
                 std::vector<Request*> one{r};
+                unstage_witness(s, one);
                 finish(s, one, HZK_ERR_NOMEM);
             } else {
                 ok.push_back(r);
@@ -253,7 +309,9 @@ void worker_loop(hzk_server* s, Pipeline* p) {
             // plain: one request through both stages, nothing overlapped
             const hzk_ctx* ec = p->ctx_h;
             int rc = stage_h(s, p->ctx_h, ok);
-            if (!rc) rc = hipStreamSynchronize(p->ctx_h->stream) == hipSuccess ? 0 : HZK_ERR_HIP;
+            const bool synced = hipStreamSynchronize(p->ctx_h->stream) == hipSuccess;
+            if (!rc && !synced) rc = HZK_ERR_HIP;
+            unstage_witness(s, ok);   // uploads are done (or the stream failed)
             if (!rc) {
                 ec = p->ctx_m;
                 rc = stage_msm(s, p->ctx_m, ok);
@@ -268,6 +326,7 @@ void worker_loop(hzk_server* s, Pipeline* p) {
         int rc1 = stage_h(s, p->ctx_h, stage1);
         int rc2 = stage_msm(s, p->ctx_m, stage2);
         if (hipStreamSynchronize(p->ctx_h->stream) != hipSuccess && !rc1) rc1 = HZK_ERR_HIP;
+        unstage_witness(s, stage1);
         for (Request* r : stage2) free_request(s, r);
         finish(s, stage2, rc2 ? rc2 : 1, p->ctx_m);
         stage2.clear();
@@ -352,6 +411,7 @@ int hzk_server_add_circuit(hzk_server* s, uint64_t seed, int log_m, int num_publ
     }
     s->circuits.push_back(g);
     s->free_bufs.emplace_back();
+    s->pinned_free.emplace_back();
     *circuit_id = (int)s->circuits.size() - 1;
     return HZK_OK;
 }
@@ -393,7 +453,8 @@ int hzk_server_submit(hzk_server* s, int circuit_id, const void* z_host, const v
     auto r = std::make_unique<Request>();
     hzk_g16* g = s->circuits[circuit_id];
     r->circuit = circuit_id;
-    r->z.assign((const uint8_t*)z_host, (const uint8_t*)z_host + g->num_vars * 32);
+    r->zbytes = g->num_vars * 32;
+    stage_witness(s, r.get(), z_host);
     std::memcpy(r->rs, rs_host, 64);
     r->footprint = g16_request_footprint(g);
     std::lock_guard<std::mutex> lk(s->mu);
@@ -418,7 +479,10 @@ int hzk_server_wait(hzk_server* s, uint64_t request_id, double timeout_s, void* 
     int st = r->status;
     if (st == 1 && proof_out) std::memcpy(proof_out, r->proof, 256);
     if (latency_s) *latency_s = st != 0 ? r->t_done - r->t_arrive : 0.0;
-    if (st != 0) s->all.erase(it);
+    if (st != 0) {
+        unstage_witness_locked(s, r);
+        s->all.erase(it);
+    }
     return st;
 }
 
@@ -450,9 +514,14 @@ void hzk_server_destroy(hzk_server* s) {
     }
     for (auto& p : s->pipes)
         if (p->th.joinable()) p->th.join();
-    for (auto& kv : s->all) free_request(s, kv.second.get());
+    for (auto& kv : s->all) {
+        free_request(s, kv.second.get());
+        if (kv.second->zp) (void)hipHostFree(kv.second->zp);
+    }
     for (auto& fl : s->free_bufs)
         for (BufSet& b : fl) release(b);
+    for (auto& fl : s->pinned_free)
+        for (void* p : fl) (void)hipHostFree(p);
     for (hzk_g16* g : s->circuits) hzk_g16_free(g);
     destroy_pipes(s);
     delete s;
