@@ -275,8 +275,9 @@ int hzk_msm_tune_get(hzk_ctx* ctx, int group, int log_n, int* c, uint32_t* K, ui
  * store the fastest in the context's table */
 int hzk_msm_autotune(hzk_ctx* ctx, int group, int log_min, int log_max, double budget_ms);
 /* The same for the throughput regime: the plans used by the instances of a
- * batch of two or more MSMs whose digit count (under the single-MSM plans)
- * reaches 2^26, which fills the GPU without help from the plan; typically
+ * wide batch (two or more MSMs whose digit count under the single-MSM plans
+ * reaches 2^26, or at least 8 G1 / 16 G2 MSMs with 2^24 digits), which fills
+ * the GPU without help from the plan; typically
  * fewer window bits (fewer buckets to reduce), longer chunks and runs. The
  * autotuner times each class as a batch of 2^26 / (16 n) instances (2..256). */
 int hzk_msm_batch_tune_set(hzk_ctx* ctx, int group, int log_n, int c, uint32_t K, uint32_t g);

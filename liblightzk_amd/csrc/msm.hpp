@@ -24,10 +24,21 @@ struct MsmPlan {
 // Plan for n points: autotuned entry for the size class if one is loaded,
 // else the built-in heuristic; force_c / force_g override (0 = automatic).
 MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes);
-// plan regimes: an MSM with the GPU to itself, or an instance of a batch
-// whose digits total at least MSM_WIDE_BATCH_ENTRIES (two or more instances)
+// plan regimes: an MSM with the GPU to itself, or an instance of a wide
+// batch: two or more instances whose digits total at least
+// MSM_WIDE_BATCH_ENTRIES, or many instances (MSM_WIDE_BATCH_COUNT, G1 / G2)
+// totalling MSM_WIDE_BATCH_MIN_ENTRIES (measured on served-like batches: a
+// batch of 16 G1 or 16 G2 MSMs of mixed sizes already runs faster on the
+// throughput plans, while the 4 G1 MSMs of one 2^18 proof do not)
 constexpr int MSM_REGIME_SINGLE = 0, MSM_REGIME_BATCH = 1;
 constexpr uint64_t MSM_WIDE_BATCH_ENTRIES = (uint64_t)1 << 26;
+constexpr uint64_t MSM_WIDE_BATCH_MIN_ENTRIES = (uint64_t)1 << 24;
+constexpr size_t MSM_WIDE_BATCH_COUNT[2] = {8, 16};
+inline bool msm_wide_batch(int g2, size_t count, uint64_t entries) {
+    if (count < 2) return false;
+    return entries >= MSM_WIDE_BATCH_ENTRIES ||
+           (count >= MSM_WIDE_BATCH_COUNT[g2 ? 1 : 0] && entries >= MSM_WIDE_BATCH_MIN_ENTRIES);
+}
 MsmPlan msm_plan_ctx(hzk_ctx* ctx, int g2, size_t n, int force_c, int regime = MSM_REGIME_SINGLE);
 // the built-in plan of a regime (no context overrides)
 MsmPlan msm_plan_builtin(int g2, size_t n, int regime, size_t lds_bytes);

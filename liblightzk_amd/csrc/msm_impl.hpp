@@ -356,9 +356,9 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.inst.resize(count);
     L.begin.assign((size_t)SP_COUNT * (count + 1), 0);
     const int g2 = sizeof(F) == 32 ? 0 : 1;
-    // regime: a batch of two or more instances whose digits (under the
-    // single-MSM plans) total MSM_WIDE_BATCH_ENTRIES fills the GPU by itself,
-    // so its instances take the throughput plans (fewer buckets, longer runs)
+    // regime: a wide batch (msm_wide_batch, digits counted under the
+    // single-MSM plans) fills the GPU by itself, so its instances take the
+    // throughput plans (fewer buckets, longer runs)
     int regime = ctx->force_regime;
     if (regime < 0) {
         uint64_t e = 0;
@@ -366,7 +366,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
             const MsmPlan q = msm_plan_ctx(ctx, g2, jobs[i].n, (int)jobs[i].win_siz, MSM_REGIME_SINGLE);
             e += (uint64_t)jobs[i].n * (uint64_t)q.W;
         }
-        regime = count >= 2 && e >= MSM_WIDE_BATCH_ENTRIES ? MSM_REGIME_BATCH : MSM_REGIME_SINGLE;
+        regime = msm_wide_batch(g2, count, e) ? MSM_REGIME_BATCH : MSM_REGIME_SINGLE;
     }
     L.regime = regime;
     for (size_t i = 0; i < count; ++i) {
