@@ -75,18 +75,27 @@ __global__ void __launch_bounds__(256) k_msm_digits(MsmBatchDev b, uint32_t* __r
         uint32_t* my = limbs[threadIdx.x];
         for (int w = 0; w < 8; ++w) my[w] = s.l[w];
         my[8] = 0;
-        const uint32_t c = d.c, NB = d.NB, mask = (1u << c) - 1;
         uint32_t carry = 0;
         uint32_t* out = digits + ebeg[ii] + idx;
+        // window j: bits [lo, lo + wj), wj = wb + 1 for j < wx, then wb (<= c);
+        // digits in [-2^(wj-1), 2^(wj-1)]
+        uint32_t lo = 0, wj = d.wb + (d.wx ? 1u : 0u);
+        uint32_t half = 1u << (wj - 1), mask = (1u << wj) - 1;
         for (uint32_t j = 0; j < d.W * d.Lg; ++j) {   // level tables: windows past Wd pad the last set
             uint32_t dg = 0;
+            if (j == d.wx && d.wx) {
+                wj = d.wb;
+                half = 1u << (wj - 1);
+                mask = (1u << wj) - 1;
+            }
             if (j < d.Wd) {
-                uint32_t lo = j * c, w = lo >> 5, sh = lo & 31;
+                uint32_t w = lo >> 5, sh = lo & 31;
                 uint64_t v = (uint64_t)my[w] | ((uint64_t)my[w + 1] << 32);
                 uint32_t raw = ((uint32_t)(v >> sh) & mask) + carry;
-                if (raw > NB) { carry = 1; dg = (2 * NB - raw) | DIGIT_NEG; }
+                if (raw > half) { carry = 1; dg = (2 * half - raw) | DIGIT_NEG; }
                 else { carry = 0; dg = raw; }
                 if ((dg & ~DIGIT_NEG) == 0) dg = 0;
+                lo += wj;
             }
             out[(size_t)j * d.nd] = dg;   // window j of scalar idx (level tables: set j / Lg, point (j % Lg) nd + idx)
         }

@@ -71,6 +71,12 @@ struct MsmInst {
     // n = nd * Lg virtual points. Plain bases: nd = n, Wd = W, Lg = 1.
     uint32_t nd, Wd, Lg;
     uint32_t tp;           // scatter in two passes (coarse groups, then buckets)
+    // Digit window j covers bits [j*wb + min(j, wx), +wb + (j < wx)): the 255
+    // bits of a signed recoding of a scalar < 2^254 split as evenly as the W
+    // windows allow (widths c and c - 1), so the top window is not a sliver
+    // of a few bits whose digits pile into a handful of buckets. Level tables
+    // keep uniform windows of c bits (wb = c, wx = 0).
+    uint32_t wb, wx;
 };
 
 struct MsmBatchDev {

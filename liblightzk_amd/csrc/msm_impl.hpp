@@ -387,9 +387,13 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         d.nd = d.n;
         d.Wd = d.W;
         d.Lg = 1;
+        d.wb = 255 / d.W;   // balanced windows (c = ceil(255 / W) >= every width)
+        d.wx = 255 % d.W;
         if (jobs[i].levels > 1) {
             // level table: every bucket set serves Lg digit windows over nd * Lg
             // virtual points; fewer bucket sets to reduce, larger buckets
+            d.wb = d.c;   // the table's levels are 2^(c t) multiples: uniform windows
+            d.wx = 0;
             d.Lg = jobs[i].levels;
             d.W = (d.Wd + d.Lg - 1) / d.Lg;
             d.n = d.nd * d.Lg;
@@ -624,7 +628,9 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         const MsmInst& d = L.inst[i];
         HostPt acc = HostPt::inf();
         for (int j = (int)d.W - 1; j >= 0; --j) {
-            for (uint32_t q = 0; q < d.c * d.Lg; ++q) acc = acc.dbl();   // a bucket set spans Lg digit windows
+            // a bucket set spans Lg digit windows of c bits; plain windows are wb or wb + 1 bits
+            const uint32_t shift = d.Lg > 1 ? d.c * d.Lg : d.wb + ((uint32_t)j < d.wx ? 1u : 0u);
+            for (uint32_t q = 0; q < shift; ++q) acc = acc.dbl();
             acc = acc.add(T[tbeg[i] + j]);
         }
         acc.store_jacobian(&out[i * 3 * sizeof(F)]);
