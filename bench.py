@@ -63,11 +63,13 @@ def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
                       f"(test/msm-check.cu:46-55 semantics), {el:.1f} s"}
 
 
-def groth16_plain(ctx, log_m, proofs):
+def groth16_plain(ctx, log_m, proofs, levels=1):
     """Plain one-proof-at-a-time GPU prover on the synthetic circuit (seed 2)."""
     import liblightzk_amd as z
     t = time.perf_counter()
     g = z.Groth16(ctx, seed=2, log_m=log_m, num_public=8)
+    if levels > 1:
+        g.set_levels(levels)
     setup_s = time.perf_counter() - t
     wits = [(g.witness(100 + i), g.rs(100 + i)) for i in range(proofs + 1)]
     g.prove(*wits[0])   # warm-up (tables, scratch)
@@ -86,8 +88,36 @@ def groth16_plain(ctx, log_m, proofs):
     lat.sort()
     return {"config": f"m=2^{log_m} multiplication-gate circuit, {g.info['num_vars']} variables",
             "proofs": proofs, "proofs_per_s": proofs / el, "p50_ms": lat[len(lat) // 2] * 1e3,
-            "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))] * 1e3, "setup_s": setup_s,
+            "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))] * 1e3, "setup_s": setup_s, "key_levels": levels,
             "kernels_ms_per_proof": {k: v[0] / proofs for k, v in rep.items()}}
+
+
+def groth16_reference_presets(ctx, ps=(39, 63, 100), proofs=5):
+    """The reference's own statements (input_scale MatMul_p_39/63/100,
+    include/protocol/groth16.cuh:11-46: the MatMul R1CS of
+    xpu-spmat.cuh:33-65) proved one at a time by the plain prover, each with a
+    satisfying seeded witness."""
+    import liblightzk_amd as z
+    out = {}
+    for p in ps:
+        t = time.perf_counter()
+        g = z.Groth16.matmul(ctx, 2, p)
+        setup_s = time.perf_counter() - t
+        wits = [(g.witness(200 + i), g.rs(200 + i)) for i in range(proofs + 1)]
+        g.prove(*wits[0])
+        lat = []
+        t0 = time.perf_counter()
+        for zz, rs in wits[1:]:
+            a = time.perf_counter()
+            g.prove(zz, rs)
+            lat.append(time.perf_counter() - a)
+        el = time.perf_counter() - t0
+        info = g.info
+        g.close()
+        out[f"MatMul_p_{p}"] = {"M": info["M"], "k": info["num_public"], "N": info["num_vars"] - 1,
+                                "proofs": proofs, "proofs_per_s": proofs / el, "p50_ms": _pct(lat, 0.5) * 1e3,
+                                "p99_ms": _pct(lat, 0.99) * 1e3, "setup_s": setup_s}
+    return out
 
 
 def batched_forms(ctx, count=128, seed=4, reps=3):
@@ -211,7 +241,8 @@ def _pct(xs, q):
     return xs[min(len(xs) - 1, int(len(xs) * q))]
 
 
-def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=2):
+def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=2, levels=1,
+                   modes=("sequential", "continuous")):
     """Mixed-size request stream served two ways by one hzk_server holding all
     circuits' keys: continuous batching (mode 0) and the plain sequential
     prover through the same queue (mode 1). Requests arrive as a seeded Poisson
@@ -223,6 +254,9 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
     srv = z.ProofServer(0, max_batch=max_batch, mode="sequential", pipelines=pipelines)
     t = time.perf_counter()
     cids = [srv.add_circuit(40 + lg, lg, 8) for lg in logs]
+    if levels > 1:
+        for cid in cids:
+            srv.set_levels(cid, levels)
     setup_s = time.perf_counter() - t
     log(f"served: setup of {len(logs)} circuits {setup_s:.1f} s")
     rng = np.random.default_rng(seed)
@@ -275,19 +309,21 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
 
     res = {}
     proofs = {}
-    for mode in ("sequential", "continuous"):
+    for mode in modes:
         st0 = srv.stats()
         proofs[mode], lat, el = run(mode)
         st1 = srv.stats()
         res[mode] = {"proofs_per_s": requests / el, "p50_ms": _pct(lat, 0.5) * 1e3, "p99_ms": _pct(lat, 0.99) * 1e3,
                      "wall_s": el, "steps": st1["steps"] - st0["steps"], "max_inflight": st1["max_inflight"]}
         log(f"served: {mode} {res[mode]['proofs_per_s']:.1f} proofs/s p99 {res[mode]['p99_ms']:.0f} ms")
-    same = all(np.array_equal(a, b) for a, b in zip(proofs["sequential"], proofs["continuous"]))
     srv.close()
+    if len(res) < 2:
+        return {"requests": requests, "key_levels": levels, **res}
+    same = all(np.array_equal(a, b) for a, b in zip(proofs["sequential"], proofs["continuous"]))
     return {"circuits": [f"m=2^{lg}" for lg in logs], "weights": list(weights), "requests": requests,
             "arrivals": f"Poisson, seed {seed}, rate {rate:.1f}/s = {rate_mult}x plain capacity {capacity:.1f}/s",
             "plain_ms_per_circuit": [x * 1e3 for x in plain_t], "setup_s": setup_s, "max_batch": max_batch,
-            "pipelines": pipelines,
+            "pipelines": pipelines, "key_levels": levels,
             "sequential": res["sequential"], "continuous": res["continuous"],
             "speedup_proofs_per_s": res["continuous"]["proofs_per_s"] / res["sequential"]["proofs_per_s"],
             "proofs_identical_across_modes": bool(same)}
@@ -303,13 +339,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--g16-log-m", type=int, default=18)
     ap.add_argument("--g16-proofs", type=int, default=5)
+    ap.add_argument("--g16-levels", type=int, default=1,
+                    help="fixed-base level tables per proving key (levels per bucket set; 1 = plain key arrays)")
     ap.add_argument("--serve-logs", default="12,16,18,20")
     ap.add_argument("--serve-weights", default="40,30,20,10")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate-mult", type=float, default=4.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-pipelines", type=int, default=2)
+    ap.add_argument("--serve-modes", default="sequential,continuous",
+                    help="served-leg modes to run (both are needed for the speedup figure)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-presets", action="store_true", help="skip the reference's MatMul p = 39/63/100 proofs")
     ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -398,8 +439,11 @@ def main():
     }
     log(f"msm: {value:.1f} Mpoints/s")
     if args.g16_proofs > 0:
-        out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs)
+        out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs, args.g16_levels)
         log(f"groth16 plain: {out['groth16_plain']['proofs_per_s']:.1f} proofs/s")
+    if not args.no_presets:
+        out["groth16_reference_presets"] = groth16_reference_presets(ctx)
+        log("reference presets: " + ", ".join(f"{k} {v['proofs_per_s']:.1f}/s" for k, v in out["groth16_reference_presets"].items()))
     if not args.no_batched:
         out["batched"] = batched_forms(ctx)
         log(f"batched: msm x{out['batched']['msm']['speedup']:.2f}, ntt x{out['batched']['ntt']['speedup']:.2f}")
@@ -410,7 +454,8 @@ def main():
         out["groth16_served"] = groth16_served([int(v) for v in args.serve_logs.split(",")],
                                                [float(v) for v in args.serve_weights.split(",")],
                                                args.serve_requests, args.serve_rate_mult, args.serve_max_batch,
-                                               pipelines=args.serve_pipelines)
+                                               pipelines=args.serve_pipelines, levels=args.g16_levels,
+                                               modes=args.serve_modes.split(","))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scalars, d_b, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -203,6 +203,21 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
 int hzk_g16_set_levels(hzk_ctx* ctx, hzk_g16* g, int levels);
 /* device pointer / count of a key array: 0 A1, 1 B1, 2 B2 (G2), 3 L1, 4 H1 */
 int hzk_g16_key_array(const hzk_g16* g, int which, void** dev_ptr, size_t* count);
+/* the reference's own statement: Groth16(input_scale) + Setup over the MatMul
+ * R1CS of xpu::spmat (include/data-structure/xpu-spmat.cuh:33-65,
+ * include/protocol/groth16.cuh:11-16, :39-71) for p x p matrices: k = 3p^2
+ * public inputs (X, Y, X*Y), N = 3p^2 + p^3 variables besides z_0,
+ * M = next_pow2(p^3 + p^2) rows, all coefficients one (input_scale p = 39, 63,
+ * 100 give M = 2^16, 2^18, 2^20). hzk_g16_witness draws X and Y from the seed
+ * and computes the partial products and X*Y forward. 2 <= p <= 255. */
+int hzk_g16_setup_matmul(hzk_ctx* ctx, uint64_t seed, int p, hzk_g16** out);
+/* proving key and R1CS to / from a file (no counterpart in the reference,
+ * which regenerates random keys per run, groth16.cuh:84-122): the loaded key
+ * proves byte-identically to the saved one. The load validates magic, version,
+ * header consistency, file size, CSR bounds and a checksum of the whole file
+ * and fails with HZK_ERR_INVALID naming the defect. */
+int hzk_g16_save(hzk_ctx* ctx, const hzk_g16* g, const char* path);
+int hzk_g16_load(hzk_ctx* ctx, const char* path, hzk_g16** out);
 
 /* ---- proof server (no counterpart in the reference, which proves one
  * statement per process, test/groth16.cu:3-8): several circuits' proving keys
@@ -224,6 +239,11 @@ void hzk_server_destroy(hzk_server* s);
 const char* hzk_server_last_error(const hzk_server* s);
 /* set up a circuit (as hzk_g16_setup) before hzk_server_start */
 int hzk_server_add_circuit(hzk_server* s, uint64_t seed, int log_m, int num_public, int* circuit_id);
+/* add a circuit from a proving-key file (hzk_g16_save), before hzk_server_start */
+int hzk_server_add_circuit_file(hzk_server* s, const char* path, int* circuit_id);
+/* prove the circuit over fixed-base level tables (hzk_g16_set_levels), before
+ * hzk_server_start; proofs are identical either way */
+int hzk_server_set_levels(hzk_server* s, int circuit_id, int levels);
 /* the circuit's hzk_g16 (witness generation, info); owned by the server */
 int hzk_server_circuit(hzk_server* s, int circuit_id, hzk_g16** out);
 /* device bytes one request of the circuit is charged against the budget */

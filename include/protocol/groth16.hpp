@@ -1,12 +1,13 @@
 // Drop-in for the reference's protocol class (include/protocol/groth16.cuh:
-// 11-229). input_scale keeps its names and domain sizes (M = 2^16, 2^18,
-// 2^20); the statement is this library's seeded multiplication-gate circuit
-// of that domain size (the reference's MatMul generator is not part of this
-// library). Setup(win_siz) builds the circuit and the proving key, resident
-// in HBM; Prove() proves a witness drawn from the next seed and keeps the
-// proof (A | B | C affine, 256 bytes) instead of printing it
-// (groth16.cuh:219-221). Setup's window size is a tuning hint: results do not
-// depend on it.
+// 11-229). input_scale keeps its names and its statement: the MatMul R1CS of
+// xpu::spmat (include/data-structure/xpu-spmat.cuh:33-65) for p = 39, 63,
+// 100 (M = 2^16, 2^18, 2^20; k = 3p^2 public inputs, N = 3p^2 + p^3).
+// Setup(win_siz) builds the circuit and the proving key from seeded toxic
+// waste, resident in HBM; Prove() proves a satisfying witness (seeded X and
+// Y, their product computed forward; the reference draws a random,
+// unsatisfying z, groth16.cuh:124-130) and keeps the proof (A | B | C
+// affine, 256 bytes) instead of printing it (groth16.cuh:219-221). Setup's
+// window size is a tuning hint: results do not depend on it.
 #pragma once
 #include <array>
 #include <cstdint>
@@ -18,7 +19,7 @@ enum class input_scale { MatMul_p_39, MatMul_p_63, MatMul_p_100 };
 class Groth16 {
 public:
     explicit Groth16(input_scale is, uint64_t seed = 2) : seed_(seed) {
-        log_m_ = is == input_scale::MatMul_p_39 ? 16 : (is == input_scale::MatMul_p_63 ? 18 : 20);
+        p_ = is == input_scale::MatMul_p_39 ? 39 : (is == input_scale::MatMul_p_63 ? 63 : 100);
     }
     ~Groth16() { hzk_g16_free(g_); }
     Groth16(const Groth16&) = delete;
@@ -28,7 +29,7 @@ public:
         (void)win_siz;
         hzk_g16_free(g_);
         g_ = nullptr;
-        lightzk::check(hzk_g16_setup(lightzk::context(), seed_, log_m_, 8, &g_));
+        lightzk::check(hzk_g16_setup_matmul(lightzk::context(), seed_, p_, &g_));
         uint64_t info[8];
         hzk_g16_info(g_, info);
         num_vars_ = info[1];
@@ -50,7 +51,7 @@ public:
 
 private:
     uint64_t seed_;
-    int log_m_;
+    int p_;
     hzk_g16* g_ = nullptr;
     size_t num_vars_ = 0;
     uint64_t wseed_ = 0;

@@ -16,6 +16,7 @@ include/data-structure/xpu-vector.cuh:10-101). Every call runs on the GPU;
 there is no CPU fallback, and a missing library raises HzkError.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -281,10 +282,15 @@ class Groth16:
     Groth16(ctx, seed, log_m) builds the synthetic circuit and runs Setup (keys
     resident in HBM); prove(z, rs) returns the 256-byte proof A | B | C."""
 
-    def __init__(self, ctx, seed, log_m, num_public=8):
+    def __init__(self, ctx, seed, log_m=None, num_public=8, matmul_p=None, path=None, _handle=None):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        ctx._check(load().hzk_g16_setup(ctx.handle, seed, log_m, num_public, ctypes.byref(h)))
+        if path is not None:      # proving key + R1CS from a file (hzk_g16_load)
+            ctx._check(load().hzk_g16_load(ctx.handle, os.fsencode(path), ctypes.byref(h)))
+        elif matmul_p is not None:  # the reference's MatMul statement (input_scale p)
+            ctx._check(load().hzk_g16_setup_matmul(ctx.handle, seed, matmul_p, ctypes.byref(h)))
+        else:
+            ctx._check(load().hzk_g16_setup(ctx.handle, seed, log_m, num_public, ctypes.byref(h)))
         self.h = h.value
         info = (ctypes.c_uint64 * 8)()
         load().hzk_g16_info(self.h, info)
@@ -324,6 +330,19 @@ class Groth16:
         """Prove over level tables with `levels` levels per bucket set (<= 1: plain key arrays)."""
         self.ctx._check(load().hzk_g16_set_levels(self.ctx.handle, self.h, levels))
 
+    @classmethod
+    def matmul(cls, ctx, seed, p):
+        """The reference's MatMul circuit for p x p matrices (input_scale p = 39 / 63 / 100)."""
+        return cls(ctx, seed, matmul_p=p)
+
+    @classmethod
+    def load_file(cls, ctx, path):
+        """Proving key and R1CS written by save()."""
+        return cls(ctx, 0, path=path)
+
+    def save(self, path):
+        self.ctx._check(load().hzk_g16_save(self.ctx.handle, self.h, os.fsencode(path)))
+
     def key_array(self, which):
         p, n = ctypes.c_void_p(), ctypes.c_size_t()
         self.ctx._check(load().hzk_g16_key_array(self.h, which, ctypes.byref(p), ctypes.byref(n)))
@@ -362,6 +381,21 @@ class ProofServer:
         keys = ["M", "num_vars", "num_public", "num_gates", "nnzA", "nnzB", "nnzC", "key_bytes"]
         self.circuits.append((g.value, {k: int(v) for k, v in zip(keys, info)}))
         return cid.value
+
+    def add_circuit_file(self, path):
+        cid = ctypes.c_int()
+        self._check(load().hzk_server_add_circuit_file(self.h, os.fsencode(path), ctypes.byref(cid)))
+        g = ctypes.c_void_p()
+        self._check(load().hzk_server_circuit(self.h, cid.value, ctypes.byref(g)))
+        info = (ctypes.c_uint64 * 8)()
+        load().hzk_g16_info(g.value, info)
+        keys = ["M", "num_vars", "num_public", "num_gates", "nnzA", "nnzB", "nnzC", "key_bytes"]
+        self.circuits.append((g.value, {k: int(v) for k, v in zip(keys, info)}))
+        return cid.value
+
+    def set_levels(self, cid, levels):
+        """Level tables for a circuit's key (before start); proofs are unchanged."""
+        self._check(load().hzk_server_set_levels(self.h, cid, levels))
 
     def witness(self, cid, wseed):
         g, info = self.circuits[cid]

@@ -73,6 +73,11 @@ def load():
         "hzk_g16_prove": (i32, [vp, vp, vp, vp, vp]),
         "hzk_g16_key_array": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(sz)]),
         "hzk_g16_set_levels": (i32, [vp, vp, i32]),
+        "hzk_g16_setup_matmul": (i32, [vp, ctypes.c_uint64, i32, ctypes.POINTER(vp)]),
+        "hzk_g16_save": (i32, [vp, vp, ctypes.c_char_p]),
+        "hzk_g16_load": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(vp)]),
+        "hzk_server_add_circuit_file": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(i32)]),
+        "hzk_server_set_levels": (i32, [vp, i32, i32]),
         "hzk_server_create": (i32, [i32, sz, i32, i32, ctypes.POINTER(vp)]),
         "hzk_server_destroy": (None, [vp]),
         "hzk_server_set_pipelines": (i32, [vp, i32]),

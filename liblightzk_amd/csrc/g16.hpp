@@ -16,7 +16,11 @@ struct HostCsr {
 
 struct hzk_g16 {
     int log_m = 0;
-    size_t M = 0, num_vars = 0, num_public = 0, num_gates = 0;
+    int kind = 0;   // 0 multiplication-gate DAG, 1 the reference's MatMul circuit (p = mm_p)
+    int mm_p = 0;
+    // rows 0..num_gates-1 each define the variable of their single C entry;
+    // z[1..num_inputs] are the seeded inputs
+    size_t M = 0, num_vars = 0, num_public = 0, num_gates = 0, num_inputs = 0;
     hzk::HostCsr m[3];
     // device copies
     void* d_val[3] = {nullptr, nullptr, nullptr};
@@ -57,5 +61,7 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph);
 void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof);
 // device bytes one request needs beyond its buffers (MSM scratch, generous)
 size_t g16_request_footprint(const hzk_g16* g);
+// allocate the plain prover's buffers (z, a, b, c) and upload 1/(g^M - 1)
+int g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g);
 
 }  // namespace hzk

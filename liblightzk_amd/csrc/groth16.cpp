@@ -100,6 +100,43 @@ void build_circuit(hzk_g16* g, uint64_t seed) {
     }
 }
 
+// The reference's MatMul R1CS for p x p matrices (xpu-spmat.cuh:33-65):
+// variables z_0 = 1, X at 1 + a p + b, Y at 1 + p^2 + a p + b, the product
+// matrix Q at 1 + 2 p^2 + a p + b, and p^3 partial products t_i at
+// 1 + 3 p^2 + i. Row i < p^3 (i = i1 p^2 + i2 p + i3) is
+// X[i2][i1] * Y[i1][i3] = t_i; row p^3 + i (i = i2 p + i3) is
+// (sum_k t_{k p^2 + i}) * z_0 = Q[i2][i3]; rows up to M = next_pow2(p^3 + p^2)
+// are empty. All coefficients are one. The public inputs are X, Y and Q
+// (3 p^2 of them, the reference's k).
+void build_circuit_matmul(hzk_g16* g) {
+    const size_t p = (size_t)g->mm_p, p2 = p * p, p3 = p2 * p;
+    HostCsr &A = g->m[0], &B = g->m[1], &C = g->m[2];
+    for (HostCsr* mm : {&A, &B, &C}) {
+        mm->row_ptr.reserve(g->M);
+        mm->col.reserve(2 * p3);
+        mm->val.reserve(2 * p3);
+    }
+    auto put = [](HostCsr& mm, uint64_t col) {
+        mm.col.push_back(col);
+        mm.val.push_back(Fr::one());
+    };
+    for (size_t i = 0; i < p3; ++i) {
+        const size_t i1 = i / p2, i2 = (i % p2) / p, i3 = i % p;
+        for (HostCsr* mm : {&A, &B, &C}) mm->row_ptr.push_back(mm->col.size());
+        put(A, 1 + i2 * p + i1);
+        put(B, 1 + p2 + i1 * p + i3);
+        put(C, 1 + 3 * p2 + i);
+    }
+    for (size_t i = 0; i < p2; ++i) {
+        for (HostCsr* mm : {&A, &B, &C}) mm->row_ptr.push_back(mm->col.size());
+        for (size_t k = 0; k < p; ++k) put(A, 1 + 3 * p2 + k * p2 + i);
+        put(B, 0);
+        put(C, 1 + 2 * p2 + i);
+    }
+    for (size_t r = p3 + p2; r < g->M; ++r)
+        for (HostCsr* mm : {&A, &B, &C}) mm->row_ptr.push_back(mm->col.size());
+}
+
 Fr row_dot(const HostCsr& m, size_t rows, size_t r, const Fr* z) {
     size_t b = m.row_ptr[r], e = r + 1 < rows ? m.row_ptr[r + 1] : m.col.size();
     Fr acc = Fr::zero();
@@ -187,7 +224,8 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
 }
 
 int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
-    build_circuit(g, seed);
+    if (g->kind == 1) build_circuit_matmul(g);
+    else build_circuit(g, seed);
     // device CSR
     for (int i = 0; i < 3; ++i) {
         int rc = dev_upload(ctx, g, g->m[i].val.data(), g->m[i].val.size() * 32, &g->d_val[i]);
@@ -245,9 +283,22 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
     canon(delta, c); g->delta1 = G.mul(c); g->delta2 = G2.mul(c);
     canon(gamma, c); g->gamma2 = G2.mul(c);
     g->IC.resize(l + 1);
-    for (size_t i = 0; i <= l; ++i) { canon(IC[i], c); g->IC[i] = G.mul(c); }
-    // per-proof buffers and 1/(g^M - 1) for the coset division
-    rc = dev_alloc(ctx, g, N1 * 32, &g->z);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i <= (long)l; ++i) {
+        uint64_t ci[4];
+        IC[i].canonical(ci);
+        g->IC[i] = G.mul(ci);
+    }
+    return g16_prover_buffers(ctx, g);
+}
+
+}  // namespace
+
+// Buffers of the plain prover (z and the three evaluation vectors) and
+// 1/(g^M - 1) for the coset division.
+int hzk::g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g) {
+    const size_t M = g->M;
+    int rc = dev_alloc(ctx, g, g->num_vars * 32, &g->z);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->a);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->b);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->c);
@@ -255,8 +306,6 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
     Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
     return dev_upload(ctx, g, zi.v, 32, &g->zinv);
 }
-
-}  // namespace
 
 namespace hzk {
 
@@ -402,7 +451,39 @@ int hzk_g16_setup(hzk_ctx* ctx, uint64_t seed, int log_m, int num_public, hzk_g1
     g->M = (size_t)1 << log_m;
     g->num_public = num_public;
     g->num_gates = g->M - (num_public + 1);
+    g->num_inputs = NUM_INPUTS;
     g->num_vars = 1 + NUM_INPUTS + g->num_gates;
+    g->device = ctx->device;
+    int rc = setup(ctx, g, seed);
+    if (rc) {
+        hzk_g16_free(g);
+        return rc;
+    }
+    *out = g;
+    return HZK_OK;
+}
+
+// The reference's MatMul statement (Groth16(input_scale) + Setup,
+// groth16.cuh:39-71): k = 3 p^2 public inputs, N = 3 p^2 + p^3 variables
+// besides z_0, M = next_pow2(p^3 + p^2) rows.
+int hzk_g16_setup_matmul(hzk_ctx* ctx, uint64_t seed, int p, hzk_g16** out) {
+    if (!ctx || !out) return HZK_ERR_INVALID;
+    *out = nullptr;
+    HZK_CHECK(ctx, p >= 2 && p <= 255, HZK_ERR_INVALID, "g16_setup_matmul: p %d out of [2, 255]", p);
+    const size_t pp = (size_t)p, rows = pp * pp * pp + pp * pp;
+    int log_m = 0;
+    while (((size_t)1 << log_m) < rows) ++log_m;
+    log_m = std::max(log_m, 4);
+    HZK_CHECK(ctx, log_m <= 24, HZK_ERR_INVALID, "g16_setup_matmul: p %d needs a domain above 2^24", p);
+    hzk_g16* g = new hzk_g16();
+    g->kind = 1;
+    g->mm_p = p;
+    g->log_m = log_m;
+    g->M = (size_t)1 << log_m;
+    g->num_public = 3 * pp * pp;
+    g->num_gates = rows;
+    g->num_inputs = 2 * pp * pp;
+    g->num_vars = 1 + 3 * pp * pp + pp * pp * pp;
     g->device = ctx->device;
     int rc = setup(ctx, g, seed);
     if (rc) {
@@ -434,11 +515,13 @@ int hzk_g16_info(const hzk_g16* g, uint64_t* out8) {
 int hzk_g16_witness(const hzk_g16* g, uint64_t wseed, void* z_host) {
     if (!g || !z_host) return HZK_ERR_INVALID;
     Fr* z = (Fr*)z_host;
+    std::memset((void*)z, 0, g->num_vars * 32);
     z[0] = Fr::one();
     synth::Xoshiro rng(wseed, 3);
-    for (int i = 1; i <= NUM_INPUTS; ++i) z[i] = fr_draw(rng);
-    for (size_t gate = 0; gate < g->num_gates; ++gate)
-        z[1 + NUM_INPUTS + gate] = row_dot(g->m[0], g->M, gate, z) * row_dot(g->m[1], g->M, gate, z);
+    for (size_t i = 1; i <= g->num_inputs; ++i) z[i] = fr_draw(rng);
+    // forward: row r defines the variable of its single C entry (coefficient one)
+    for (size_t r = 0; r < g->num_gates; ++r)
+        z[g->m[2].col[g->m[2].row_ptr[r]]] = row_dot(g->m[0], g->M, r, z) * row_dot(g->m[1], g->M, r, z);
     return HZK_OK;
 }
 

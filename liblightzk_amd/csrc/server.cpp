@@ -416,6 +416,32 @@ int hzk_server_add_circuit(hzk_server* s, uint64_t seed, int log_m, int num_publ
     return HZK_OK;
 }
 
+// Add a circuit from a proving-key file written by hzk_g16_save.
+int hzk_server_add_circuit_file(hzk_server* s, const char* path, int* circuit_id) {
+    if (!s || !circuit_id || s->running) return HZK_ERR_INVALID;
+    hzk_g16* g = nullptr;
+    hzk_ctx* ctx = s->pipes[0]->ctx_m;
+    int rc = hzk_g16_load(ctx, path, &g);
+    if (rc) {
+        s->last_error = hzk_last_error(ctx);
+        return rc;
+    }
+    s->circuits.push_back(g);
+    s->free_bufs.emplace_back();
+    s->pinned_free.emplace_back();
+    *circuit_id = (int)s->circuits.size() - 1;
+    return HZK_OK;
+}
+
+// Fixed-base level tables for a circuit's key (hzk_g16_set_levels), before start.
+int hzk_server_set_levels(hzk_server* s, int circuit_id, int levels) {
+    if (!s || s->running || circuit_id < 0 || circuit_id >= (int)s->circuits.size()) return HZK_ERR_INVALID;
+    hzk_ctx* ctx = s->pipes[0]->ctx_m;
+    int rc = hzk_g16_set_levels(ctx, s->circuits[circuit_id], levels);
+    if (rc) s->last_error = hzk_last_error(ctx);
+    return rc;
+}
+
 int hzk_server_circuit(hzk_server* s, int circuit_id, hzk_g16** out) {
     if (!s || !out || circuit_id < 0 || circuit_id >= (int)s->circuits.size()) return HZK_ERR_INVALID;
     *out = s->circuits[circuit_id];

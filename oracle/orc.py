@@ -61,6 +61,11 @@ def lib():
             "orc_circuit_witness": (i32, [u64, i32, i32, u64, U64P]),
             "orc_r1cs_satisfied": (i32, [u64, i32, i32, U64P]),
             "orc_g16_new": (vp, [u64, i32, i32]),
+            "orc_circuit_sizes_k": (i32, [i32, u64, i32, i32, U64P]),
+            "orc_circuit_csr_k": (i32, [i32, u64, i32, i32, i32, U64P, U64P, U64P]),
+            "orc_circuit_witness_k": (i32, [i32, u64, i32, i32, u64, U64P]),
+            "orc_r1cs_satisfied_k": (i32, [i32, u64, i32, i32, U64P]),
+            "orc_g16_new_k": (vp, [i32, u64, i32, i32]),
             "orc_g16_free": (None, [vp]),
             "orc_g16_key_scalars": (sz, [vp, i32, U64P]),
             "orc_g16_key_points": (sz, [vp, i32, U64P]),
@@ -235,13 +240,51 @@ def r1cs_satisfied(seed, log_m, num_public, z):
     return bool(lib().orc_r1cs_satisfied(seed, log_m, num_public, ptr(z)))
 
 
-class Groth16:
-    """Oracle Groth16 instance (circuit + key scalars; points on demand)."""
+def matmul_sizes(p):
+    """Sizes of the reference's MatMul circuit for p x p matrices."""
+    out = np.zeros(7, dtype=np.uint64)
+    _chk(lib().orc_circuit_sizes_k(1, 0, p, 0, ptr(out)))
+    keys = ["M", "num_vars", "num_public", "num_gates", "nnzA", "nnzB", "nnzC"]
+    return {k: int(v) for k, v in zip(keys, out)}
 
-    def __init__(self, seed, log_m, num_public=8):
-        self.seed, self.log_m, self.num_public = seed, log_m, num_public
-        self.sizes = circuit_sizes(seed, log_m, num_public)
-        self.h = lib().orc_g16_new(seed, log_m, num_public)
+
+def matmul_csr(p, which):
+    s = matmul_sizes(p)
+    nnz = s["nnz" + "ABC"[which]]
+    row_ptr = np.zeros(s["M"], dtype=np.uint64)
+    col = np.zeros(nnz, dtype=np.uint64)
+    val = np.zeros(4 * nnz, dtype=np.uint64)
+    _chk(lib().orc_circuit_csr_k(1, 0, p, 0, which, ptr(row_ptr), ptr(col), ptr(val)))
+    return row_ptr, col, val
+
+
+def matmul_witness(p, wseed):
+    s = matmul_sizes(p)
+    z = np.zeros(4 * s["num_vars"], dtype=np.uint64)
+    _chk(lib().orc_circuit_witness_k(1, 0, p, 0, wseed, ptr(z)))
+    return z
+
+
+def matmul_satisfied(p, z):
+    return bool(lib().orc_r1cs_satisfied_k(1, 0, p, 0, ptr(z)))
+
+
+class Groth16:
+    """Oracle Groth16 instance (circuit + key scalars; points on demand).
+    matmul_p selects the reference's MatMul circuit instead of the
+    multiplication-gate DAG."""
+
+    def __init__(self, seed, log_m=None, num_public=8, matmul_p=None):
+        self.seed, self.matmul_p = seed, matmul_p
+        if matmul_p is not None:
+            self.sizes = matmul_sizes(matmul_p)
+            self.log_m = self.sizes["M"].bit_length() - 1
+            self.num_public = self.sizes["num_public"]
+            self.h = lib().orc_g16_new_k(1, seed, matmul_p, 0)
+        else:
+            self.log_m, self.num_public = log_m, num_public
+            self.sizes = circuit_sizes(seed, log_m, num_public)
+            self.h = lib().orc_g16_new(seed, log_m, num_public)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -266,6 +309,8 @@ class Groth16:
         return out
 
     def witness(self, wseed):
+        if self.matmul_p is not None:
+            return matmul_witness(self.matmul_p, wseed)
         return circuit_witness(self.seed, self.log_m, self.num_public, wseed)
 
     def h_coeffs(self, z):

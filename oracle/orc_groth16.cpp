@@ -44,6 +44,8 @@ struct Csr {
 
 struct Circuit {
     int log_M;
+    int kind = 0;   // 0 multiplication-gate DAG, 1 MatMul (p = mm_p)
+    int mm_p = 0;
     size_t M, num_vars, num_public, num_gates;
     Csr A, B, C;
 };
@@ -81,6 +83,66 @@ Circuit make_circuit(uint64_t seed, int log_M, int num_public) {
     return c;
 }
 
+// The reference's MatMul R1CS (include/data-structure/xpu-spmat.cuh:33-65,
+// sizes include/protocol/groth16.cuh:11-16): p^3 + p^2 constraint rows padded
+// with empty rows to M = next_pow2(p^3 + p^2), N + 1 = 1 + 3p^2 + p^3
+// variables, every coefficient one. Row i < p^3 with (i1, i2, i3) the base-p
+// digits of i: A = z[1 + i2 p + i1], B = z[1 + p^2 + i1 p + i3],
+// C = z[1 + 3p^2 + i]. Row p^3 + i, i < p^2: A = sum_k z[1 + 3p^2 + k p^2 + i],
+// B = z[0], C = z[1 + 2p^2 + i]. Public inputs: the first k = 3p^2 variables.
+Circuit make_circuit_matmul(int p) {
+    Circuit c;
+    const size_t P = (size_t)p, p2 = P * P, p3 = p2 * P;
+    c.kind = 1;
+    c.mm_p = p;
+    c.log_M = 0;
+    while (((size_t)1 << c.log_M) < p3 + p2) ++c.log_M;
+    if (c.log_M < 4) c.log_M = 4;
+    c.M = (size_t)1 << c.log_M;
+    c.num_public = 3 * p2;
+    c.num_gates = p3 + p2;
+    c.num_vars = 1 + 3 * p2 + p3;
+    auto row = [&](Csr& m) { m.row_ptr.push_back(m.col.size()); };
+    auto one = [&](Csr& m, uint64_t col) { m.col.push_back(col); m.val.push_back(Fr::one()); };
+    for (size_t i = 0; i < p3; ++i) {
+        size_t i1 = i / p2, i2 = (i / P) % P, i3 = i % P;
+        row(c.A); one(c.A, 1 + i2 * P + i1);
+        row(c.B); one(c.B, 1 + p2 + i1 * P + i3);
+        row(c.C); one(c.C, 1 + 3 * p2 + i);
+    }
+    for (size_t i = 0; i < p2; ++i) {
+        row(c.A);
+        for (size_t k = 0; k < P; ++k) one(c.A, 1 + 3 * p2 + k * p2 + i);
+        row(c.B); one(c.B, 0);
+        row(c.C); one(c.C, 1 + 2 * p2 + i);
+    }
+    for (size_t r = p3 + p2; r < c.M; ++r) { row(c.A); row(c.B); row(c.C); }
+    return c;
+}
+
+// MatMul witness: X then Y (p^2 each, row-major) from stream 3 of wseed,
+// t[i1 p^2 + i2 p + i3] = X[i2][i1] Y[i1][i3], Q = X Y.
+std::vector<Fr> make_witness_matmul(const Circuit& c, uint64_t wseed) {
+    const size_t P = (size_t)c.mm_p, p2 = P * P;
+    std::vector<Fr> z(c.num_vars, Fr::zero());
+    z[0] = Fr::one();
+    Rng rng(wseed, 3);
+    for (size_t i = 1; i <= 2 * p2; ++i) z[i] = rng.field<Fr>();
+    const Fr* X = &z[1];
+    const Fr* Y = &z[1 + p2];
+    for (size_t a = 0; a < P; ++a)
+        for (size_t b = 0; b < P; ++b) {
+            Fr q = Fr::zero();
+            for (size_t k = 0; k < P; ++k) {
+                Fr t = X[a * P + k] * Y[k * P + b];
+                z[1 + 3 * p2 + k * p2 + a * P + b] = t;
+                q += t;
+            }
+            z[1 + 2 * p2 + a * P + b] = q;
+        }
+    return z;
+}
+
 Fr row_dot(const Csr& m, size_t rows, size_t r, const std::vector<Fr>& z) {
     size_t b = m.row_ptr[r], e = r + 1 < rows ? m.row_ptr[r + 1] : m.col.size();
     Fr acc = Fr::zero();
@@ -89,6 +151,7 @@ Fr row_dot(const Csr& m, size_t rows, size_t r, const std::vector<Fr>& z) {
 }
 
 std::vector<Fr> make_witness(const Circuit& c, uint64_t wseed) {
+    if (c.kind == 1) return make_witness_matmul(c, wseed);
     std::vector<Fr> z(c.num_vars);
     z[0] = Fr::one();
     Rng rng(wseed, 3);
@@ -252,9 +315,64 @@ G msm_pts(const std::vector<Fr>& s, size_t s_off, const std::vector<G>& P, size_
     return msm_pippenger<G>(s[s_off].v, n, [&P](size_t i) { return P[i]; });
 }
 
+// kind 0: multiplication-gate DAG (param = log_M); kind 1: MatMul (param = p).
+bool make_any(int kind, uint64_t seed, int param, int num_public, Circuit& out) {
+    if (kind == 1) {
+        if (param < 2 || param > 255) return false;
+        out = make_circuit_matmul(param);
+        return out.log_M <= 24;
+    }
+    if (kind != 0 || param < 4 || param > 24 || num_public < 0 || (size_t)num_public + 1 >= ((size_t)1 << param)) return false;
+    out = make_circuit(seed, param, num_public);
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Circuit of either kind: sizes {M, num_vars, num_public, num_gates, nnzA, nnzB, nnzC}.
+int orc_circuit_sizes_k(int kind, uint64_t seed, int param, int num_public, uint64_t* out) {
+    Circuit c;
+    if (!make_any(kind, seed, param, num_public, c)) return -1;
+    out[0] = c.M; out[1] = c.num_vars; out[2] = c.num_public; out[3] = c.num_gates;
+    out[4] = c.A.col.size(); out[5] = c.B.col.size(); out[6] = c.C.col.size();
+    return 0;
+}
+int orc_circuit_csr_k(int kind, uint64_t seed, int param, int num_public, int which, uint64_t* row_ptr, uint64_t* col, uint64_t* val) {
+    Circuit c;
+    if (!make_any(kind, seed, param, num_public, c)) return -1;
+    const Csr& m = which == 0 ? c.A : which == 1 ? c.B : c.C;
+    std::memcpy(row_ptr, m.row_ptr.data(), 8 * m.row_ptr.size());
+    std::memcpy(col, m.col.data(), 8 * m.col.size());
+    std::memcpy(val, m.val.data(), 32 * m.val.size());
+    return 0;
+}
+int orc_circuit_witness_k(int kind, uint64_t seed, int param, int num_public, uint64_t wseed, uint64_t* z) {
+    Circuit c;
+    if (!make_any(kind, seed, param, num_public, c)) return -1;
+    std::vector<Fr> w = make_witness(c, wseed);
+    std::memcpy(z, w.data(), 32 * w.size());
+    return 0;
+}
+int orc_r1cs_satisfied_k(int kind, uint64_t seed, int param, int num_public, const uint64_t* z) {
+    Circuit c;
+    if (!make_any(kind, seed, param, num_public, c)) return -1;
+    std::vector<Fr> zz(c.num_vars);
+    std::memcpy(zz.data(), z, 32 * c.num_vars);
+    for (size_t r = 0; r < c.M; ++r)
+        if (row_dot(c.A, c.M, r, zz) * row_dot(c.B, c.M, r, zz) != row_dot(c.C, c.M, r, zz)) return 0;
+    return 1;
+}
+void* orc_g16_new_k(int kind, uint64_t seed, int param, int num_public) {
+    Circuit c;
+    if (!make_any(kind, seed, param, num_public, c)) return nullptr;
+    Oracle16* o = new Oracle16();
+    o->c = std::move(c);
+    o->ks = key_scalars(o->c, seed);
+    return o;
+}
+
 
 // Sizes of the synthetic circuit: out = {M, num_vars, num_public, num_gates, nnzA, nnzB, nnzC}.
 int orc_circuit_sizes(uint64_t seed, int log_M, int num_public, uint64_t* out) {

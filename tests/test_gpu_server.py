@@ -135,3 +135,27 @@ def test_injected_failure_is_isolated(oracles):
             assert st == 1
             assert np.array_equal(proof, oracles[8].prove_trapdoor(zz, rs))
         assert s.stats()["failed"] == 1
+
+
+def test_server_restart_from_key_files_with_level_tables(ctx, oracles, tmp_path):
+    """A server restarted from saved proving keys (hzk_server_add_circuit_file),
+    one of them proving over level tables (hzk_server_set_levels), serves a
+    mixed stream whose proofs equal the closed form."""
+    import liblightzk_amd as z
+    paths = {}
+    for lg in LOGS:
+        g = z.Groth16(ctx, seed=SEED + lg, log_m=lg, num_public=8)
+        paths[lg] = str(tmp_path / f"c{lg}.hzk")
+        g.save(paths[lg])
+        g.close()
+    s = z.ProofServer(0, max_batch=16, mode="continuous")
+    cids = {lg: s.add_circuit_file(paths[lg]) for lg in LOGS}
+    s.set_levels(cids[LOGS[-1]], 4)
+    reqs = _requests(s, cids, 12, 9)
+    with s:
+        s.start()
+        ids = [s.submit(cids[lg], zz, rs) for lg, zz, rs in reqs]
+        for (lg, zz, rs), rid in zip(reqs, ids):
+            st, proof, _ = s.wait(rid, 120)
+            assert st == 1, s.last_error()
+            assert np.array_equal(proof, oracles[lg].prove_trapdoor(zz, rs)), lg

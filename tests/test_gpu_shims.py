@@ -7,10 +7,10 @@ layout give the oracle's closed form (sum s_i k_i) G for G1 and G2; the
 pre_comp level records are 2^(gap j) P_0 with Z = 1; the NTT with the
 caller's ntt_setup table and factor equals the oracle's restatement of the
 reference operator; SpMV equals the oracle's CSR product and A o B - C
-vanishes on a satisfying witness; the protocol class's proof equals the C
-ABI prover's for the same seeds (which other tests pin to the oracle's
-closed form and pairing check); a bad call raises lightzk::error with the
-library's message."""
+vanishes on a satisfying witness; the protocol class (input_scale p = 39,
+the reference's MatMul statement) proves a witness equal to the oracle's and
+a proof equal to the C ABI prover's and to the oracle's trapdoor closed form;
+a bad call raises lightzk::error with the library's message."""
 import os
 import subprocess
 
@@ -102,10 +102,15 @@ def test_ntt_and_spmv(orc, run):
     assert not _read(d, "abc.bin").any()
 
 
-def test_protocol_class_and_errors(ctx, run):
+def test_protocol_class_and_errors(ctx, orc, run):
     import liblightzk_amd as z
-    g = z.Groth16(ctx, seed=2, log_m=16, num_public=8)
-    want = g.prove(g.witness(1), g.rs(1))
+    g = z.Groth16.matmul(ctx, 2, 39)      # input_scale::MatMul_p_39
+    zz, rs = g.witness(1), g.rs(1)
+    want = g.prove(zz, rs)
     g.close()
-    assert np.array_equal(_read(run["d"], "g16_proof.bin"), want)
+    got = _read(run["d"], "g16_proof.bin")
+    assert np.array_equal(got, want)
+    o = orc.Groth16(2, matmul_p=39)       # the reference's statement at its own size
+    assert np.array_equal(zz, o.witness(1))
+    assert np.array_equal(got, o.prove_trapdoor(zz, rs))
     assert "not a power of two" in run["status"][1] or "deg" in run["status"][1], run["status"]

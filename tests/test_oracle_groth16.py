@@ -105,3 +105,83 @@ def test_prove_verify(orc, g16):
     wrong_pub = public.copy()
     wrong_pub[4:8] = pm.int_to_limbs(pm.to_mont(12345, pm.R))
     assert not g16.verify(wrong_pub, p_honest)
+
+
+# ---- the reference's MatMul statement (xpu-spmat.cuh:33-65, groth16.cuh:11-46)
+
+# (p, k, N, M) of the reference's input_scale presets, include/protocol/groth16.cuh:39-46
+REF_PRESETS = [(39, 4563, 63882, 65536), (63, 11907, 261954, 262144), (100, 30000, 1030000, 1048576)]
+
+
+@pytest.mark.parametrize("p,k,N,M", REF_PRESETS)
+def test_matmul_sizes_equal_reference_presets(orc, p, k, N, M):
+    s = orc.matmul_sizes(p)
+    assert (s["num_public"], s["num_vars"] - 1, s["M"]) == (k, N, M)
+    # the reference's allocation sizes for val/col_idx (xpu-spmat.cuh:39-40)
+    assert (s["nnzA"], s["nnzB"], s["nnzC"]) == (2 * p ** 3, p ** 3 + p ** 2, p ** 3 + p ** 2)
+    assert s["num_gates"] == p ** 3 + p ** 2
+
+
+def test_matmul_witness_is_the_matrix_product(orc):
+    p = 5
+    z = orc.matmul_witness(p, 9)
+    v = [pm.from_mont(x, pm.R) for x in pm.elems(z)]
+    p2 = p * p
+    X = [v[1 + i * p: 1 + (i + 1) * p] for i in range(p)]
+    Y = [v[1 + p2 + i * p: 1 + p2 + (i + 1) * p] for i in range(p)]
+    Q = [v[1 + 2 * p2 + i * p: 1 + 2 * p2 + (i + 1) * p] for i in range(p)]
+    assert v[0] == 1
+    for a in range(p):
+        for b in range(p):
+            assert Q[a][b] == sum(X[a][t] * Y[t][b] for t in range(p)) % pm.R
+    assert orc.matmul_satisfied(p, z)
+    bad = z.copy()
+    bad[4 * (1 + 2 * p2)] ^= np.uint64(1)       # Q[0][0] no longer X Y
+    assert not orc.matmul_satisfied(p, bad)
+    bad = z.copy()
+    bad[4 * (1 + 3 * p2 + 7)] ^= np.uint64(1)   # one partial product
+    assert not orc.matmul_satisfied(p, bad)
+
+
+def test_matmul_csr_dense_semantics(orc):
+    """Every row of the dense A, B, C over a random assignment: rows < p^3
+    select X[i2][i1], Y[i1][i3], t_i; rows p^3 + i sum the t_{k p^2 + i},
+    select z_0 and Q_i; padding rows are empty."""
+    p = 3
+    s = orc.matmul_sizes(p)
+    p2, p3 = p * p, p ** 3
+    dense = []
+    for which in range(3):
+        rp, col, val = orc.matmul_csr(p, which)
+        ends = list(rp[1:]) + [col.size]
+        rows = [[0] * s["num_vars"] for _ in range(s["M"])]
+        for r in range(s["M"]):
+            for j in range(int(rp[r]), int(ends[r])):
+                rows[r][int(col[j])] += pm.from_mont(pm.limbs_to_int(val[4 * j: 4 * j + 4]), pm.R)
+        dense.append(rows)
+    A, B, C = dense
+    for i in range(p3):
+        i1, i2, i3 = i // p2, (i // p) % p, i % p
+        assert [c for c, x in enumerate(A[i]) if x] == [1 + i2 * p + i1] and A[i][1 + i2 * p + i1] == 1
+        assert [c for c, x in enumerate(B[i]) if x] == [1 + p2 + i1 * p + i3]
+        assert [c for c, x in enumerate(C[i]) if x] == [1 + 3 * p2 + i]
+    for i in range(p2):
+        r = p3 + i
+        assert [c for c, x in enumerate(A[r]) if x] == sorted(1 + 3 * p2 + k * p2 + i for k in range(p))
+        assert [c for c, x in enumerate(B[r]) if x] == [0]
+        assert [c for c, x in enumerate(C[r]) if x] == [1 + 2 * p2 + i]
+    for r in range(p3 + p2, s["M"]):
+        assert not any(A[r]) and not any(B[r]) and not any(C[r])
+
+
+def test_matmul_proof_verifies(orc):
+    o = orc.Groth16(3, matmul_p=3)
+    z = o.witness(21)
+    rs = o.rs(21)
+    p = o.prove(z, rs)
+    assert np.array_equal(p, o.prove_trapdoor(z, rs))
+    npub = o.sizes["num_public"]
+    assert o.verify(z[: 4 * (npub + 1)], p)
+    wrong = z[: 4 * (npub + 1)].copy()
+    wrong[4 * npub] ^= np.uint64(1)     # a different product matrix entry
+    assert not o.verify(wrong, p)
