@@ -63,6 +63,31 @@ def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
                       f"(test/msm-check.cu:46-55 semantics), {el:.1f} s"}
 
 
+def cpu_prover_baseline(ctx, log_m=16, proofs=2):
+    """Part of the cpu_baseline leg: the oracle's honest Groth16 prover (CPU
+    SpMV, radix-2 NTTs, CPU Pippenger MSMs, phase-6 assembly; the phase order
+    of include/protocol/groth16.cuh:154-222) on a bounded circuit, next to the
+    GPU plain prover on the same circuit and witnesses (proof bytes compared)."""
+    import liblightzk_amd as z
+    from oracle import orc
+    o = orc.Groth16(seed=2, log_m=log_m, num_public=8)
+    o.key_points(0)                       # key points built before the clock
+    g = z.Groth16(ctx, seed=2, log_m=log_m, num_public=8)
+    wits = [(o.witness(300 + i), o.rs(300 + i)) for i in range(proofs)]
+    t = time.perf_counter()
+    cpu = [o.prove(zz, rs) for zz, rs in wits]
+    cpu_s = (time.perf_counter() - t) / proofs
+    g.prove(*wits[0])
+    t = time.perf_counter()
+    gpu = [g.prove(zz, rs) for zz, rs in wits]
+    gpu_s = (time.perf_counter() - t) / proofs
+    g.close()
+    return {"value": 1.0 / cpu_s, "unit": "proofs/s", "cores": orc.lib().orc_num_threads(), "kind": "port",
+            "sample": f"{proofs} proofs of the m=2^{log_m} multiplication-gate circuit (CPU Pippenger MSMs)",
+            "gpu_plain_same_circuit_proofs_per_s": 1.0 / gpu_s,
+            "proofs_identical": bool(all(np.array_equal(a, b) for a, b in zip(cpu, gpu)))}
+
+
 def groth16_plain(ctx, log_m, proofs, levels=1):
     """Plain one-proof-at-a-time GPU prover on the synthetic circuit (seed 2)."""
     import liblightzk_amd as z
@@ -458,6 +483,7 @@ def main():
                                                modes=args.serve_modes.split(","))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, scalars, d_b, args.cpu_seconds)
+        out["cpu_baseline"]["groth16"] = cpu_prover_baseline(ctx)
     print(json.dumps(out), flush=True)
     ctx.close()
 
