@@ -145,6 +145,44 @@ def groth16_reference_presets(ctx, ps=(39, 63, 100), proofs=5):
     return out
 
 
+def served_msm_batches(ctx, reps=3, seed=4):
+    """The MSM stage of 32 served requests of the 40/30/20/10 mix (13 x 2^12,
+    10 x 2^16, 6 x 2^18, 3 x 2^20 constraints) launched alone: one G1 batch of
+    4 MSMs per request (128 MSMs) and one G2 batch of 1 MSM per request (32
+    MSMs), sizes 2^k, seeded uniform scalars and bases k_i*G (wide-batch plans)."""
+    import liblightzk_amd as z
+    mix = [12] * 13 + [16] * 10 + [18] * 6 + [20] * 3
+    nmax = 1 << 20
+    s = z.gen_field(seed, 0, 2 * nmax, "fr")
+    k = z.gen_field(seed, 1, nmax, "fr")
+    ds, dk = ctx.upload(s), ctx.upload(k)
+    b1, b2 = ctx.alloc(64 * nmax), ctx.alloc(128 * nmax)
+    ctx.fixbase_g1(dk, nmax, b1)
+    ctx.fixbase_g2(dk, nmax, b2)
+    out1, out2 = ctx.alloc(96 * 4 * len(mix)), ctx.alloc(192 * len(mix))
+    rng = np.random.default_rng([seed, 9])
+    j1, j2 = [], []
+    for i, lg in enumerate(mix):
+        for t in range(4):
+            j1.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b1.ptr, 1 << lg, out1.ptr + 96 * (4 * i + t)))
+        j2.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b2.ptr, 1 << lg, out2.ptr + 192 * i))
+    res = {}
+    for grp, jobs in ((1, j1), (2, j2)):
+        ctx.msm_batch(grp, jobs)   # warm-up (scratch, plans)
+        ts = []
+        for _ in range(reps):
+            ctx.sync()
+            t = time.perf_counter()
+            ctx.msm_batch(grp, jobs)
+            ctx.sync()
+            ts.append(time.perf_counter() - t)
+        pts = sum(j[2] for j in jobs)
+        res[f"g{grp}"] = {"msms": len(jobs), "points": pts, "ms": min(ts) * 1e3, "Mpoints_per_s": pts / min(ts) / 1e6}
+    for buf in (ds, dk, b1, b2, out1, out2):
+        buf.free()
+    return res
+
+
 def batched_forms(ctx, count=128, seed=4, reps=3):
     """128 G1 MSMs with n = floor(2^(10 + 10U)) and 128 Fr NTTs of 2^k, k
     uniform in 10..20 (seeded stream 6 of seed 4): each set as one batched,
@@ -375,6 +413,8 @@ def main():
     ap.add_argument("--serve-modes", default="sequential,continuous",
                     help="served-leg modes to run (both are needed for the speedup figure)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--served-batch-reps", type=int, default=3,
+                    help="timed runs of the served mix's MSM batches launched alone (0 = skip)")
     ap.add_argument("--no-presets", action="store_true", help="skip the reference's MatMul p = 39/63/100 proofs")
     ap.add_argument("--no-large", action="store_true")
     args = ap.parse_args()
@@ -469,6 +509,9 @@ def main():
     if not args.no_presets:
         out["groth16_reference_presets"] = groth16_reference_presets(ctx)
         log("reference presets: " + ", ".join(f"{k} {v['proofs_per_s']:.1f}/s" for k, v in out["groth16_reference_presets"].items()))
+    if args.served_batch_reps > 0:
+        out["served_msm_batches"] = served_msm_batches(ctx, args.served_batch_reps)
+        log("served MSM batches: " + ", ".join(f"{k} {v['ms']:.1f} ms" for k, v in out["served_msm_batches"].items()))
     if not args.no_batched:
         out["batched"] = batched_forms(ctx)
         log(f"batched: msm x{out['batched']['msm']['speedup']:.2f}, ntt x{out['batched']['ntt']['speedup']:.2f}")
