@@ -229,6 +229,12 @@ int hzk_g16_load(hzk_ctx* ctx, const char* path, hzk_g16** out) {
             if (m.row_ptr[j] > h.nnz[i] || (j && m.row_ptr[j] < m.row_ptr[j - 1])) bad = "corrupt row pointers";
         for (size_t j = 0; j < h.nnz[i] && !bad; ++j)
             if (m.col[j] >= g->num_vars) bad = "column index out of range";
+        // the witness generator evaluates row r < num_gates into the variable
+        // of its single C entry
+        for (size_t r = 0; i == 2 && r < g->num_gates && !bad; ++r) {
+            const uint64_t e = r + 1 < g->M ? m.row_ptr[r + 1] : h.nnz[i];
+            if (e != m.row_ptr[r] + 1) bad = "a constraint row without a single output entry";
+        }
         if (bad) break;
     }
     std::vector<char> stage(STAGE);

@@ -125,3 +125,43 @@ def test_key_file_round_trip_and_defects(ctx, orc, tmp_path):
             assert word in str(e.value) and "rc=-1" in str(e.value), (word, str(e.value))
     with pytest.raises(z.HzkError):
         z.Groth16.load_file(ctx, str(tmp_path / "missing.hzk"))
+
+
+def _key_checksum(data):
+    """The key file's checksum: 8-byte little-endian words w, tail zero-padded,
+    h = (h ^ w) * 0x100000001b3 + rotl(h, 29) mod 2^64 from 0xcbf29ce484222325."""
+    import struct
+    mask = (1 << 64) - 1
+    h = 0xcbf29ce484222325
+    pad = data + b"\0" * (-len(data) % 8)
+    for (w,) in struct.iter_unpack("<Q", pad):
+        h = (((h ^ w) * 0x100000001b3) + (((h << 29) | (h >> 35)) & mask)) & mask
+    return h
+
+
+def test_key_file_layout_and_semantic_checks(ctx, tmp_path):
+    """The documented layout (88-byte header, CSR sections, trailing checksum)
+    matches what hzk_g16_save writes; a file whose checksum is valid but whose
+    R1CS breaks an invariant the prover relies on is still rejected."""
+    import struct
+    import liblightzk_amd as z
+    g = z.Groth16(ctx, seed=8, log_m=7, num_public=3)
+    path = str(tmp_path / "k.hzk")
+    g.save(path)
+    info = dict(g.info)
+    g.close()
+    raw = open(path, "rb").read()
+    magic, ver, kind, mm_p, log_m, M, nv, npub, ngates, ninp, nA, nB, nC = struct.unpack("<8s4I8Q", raw[:88])
+    assert magic == b"HZKG16\0\1" and ver == 1 and kind == 0 and log_m == 7 and M == 128
+    assert (nv, npub, ngates, nA, nB, nC) == (info["num_vars"], 3, info["num_gates"], info["nnzA"], info["nnzB"], info["nnzC"])
+    assert struct.unpack("<Q", raw[-8:])[0] == _key_checksum(raw[:-8])
+    # empty C row 0 (row_ptr[1] = row_ptr[0]) with a recomputed checksum
+    offC = 88 + (M * 8 + nA * 40) + (M * 8 + nB * 40)
+    body = bytearray(raw[:-8])
+    body[offC + 8: offC + 16] = body[offC: offC + 8]
+    bad = bytes(body) + struct.pack("<Q", _key_checksum(bytes(body)))
+    p2 = str(tmp_path / "bad.hzk")
+    open(p2, "wb").write(bad)
+    with pytest.raises(z.HzkError) as e:
+        z.Groth16.load_file(ctx, p2)
+    assert "single output entry" in str(e.value)
