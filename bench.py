@@ -305,13 +305,15 @@ def _pct(xs, q):
 
 
 def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=2, levels=1,
-                   modes=("sequential", "continuous")):
+                   modes=("sequential", "continuous"), under_load_requests=150):
     """Mixed-size request stream served two ways by one hzk_server holding all
     circuits' keys: continuous batching (mode 0) and the plain sequential
     prover through the same queue (mode 1). Requests arrive as a seeded Poisson
     stream at rate_mult x the plain prover's measured capacity on this mix;
     each carries its own seeded witness (generated before the clock starts).
-    Both runs must return the same proof bytes request by request."""
+    Both runs must return the same proof bytes request by request. Then the
+    first under_load_requests requests arrive at half the plain capacity
+    (the under-load latency point), in both modes, with the same proofs."""
     import threading
     import liblightzk_amd as z
     srv = z.ProofServer(0, max_batch=max_batch, mode="sequential", pipelines=pipelines)
@@ -343,14 +345,14 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
     rate = rate_mult * capacity
     arrivals = np.cumsum(gaps / rate)
 
-    def run(mode):
+    def run(mode, count=requests, arr=arrivals):
         srv.set_mode(mode)
-        ids = [None] * requests
+        ids = [None] * count
         t0 = time.perf_counter()
 
         def client():
-            for i, (k, zz, rs) in enumerate(reqs):
-                d = t0 + arrivals[i] - time.perf_counter()
+            for i, (k, zz, rs) in enumerate(reqs[:count]):
+                d = t0 + arr[i] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
                 ids[i] = srv.submit(cids[k], zz, rs)
@@ -358,7 +360,7 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
         th = threading.Thread(target=client)
         th.start()
         out, lat = [], []
-        for i in range(requests):
+        for i in range(count):
             while ids[i] is None:
                 time.sleep(0.0005)
             st, proof, l = srv.wait(ids[i], 600)
@@ -379,6 +381,18 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
         res[mode] = {"proofs_per_s": requests / el, "p50_ms": _pct(lat, 0.5) * 1e3, "p99_ms": _pct(lat, 0.99) * 1e3,
                      "wall_s": el, "steps": st1["steps"] - st0["steps"], "max_inflight": st1["max_inflight"]}
         log(f"served: {mode} {res[mode]['proofs_per_s']:.1f} proofs/s p99 {res[mode]['p99_ms']:.0f} ms")
+    # under load: the same stream's first requests at half the plain capacity,
+    # where latency is the proving time plus any wait for a batch to form
+    under = {}
+    if len(res) == 2 and under_load_requests > 0:
+        n2 = min(requests, under_load_requests)
+        arr2 = np.cumsum(gaps[:n2] / (0.5 * capacity))
+        for mode in modes:
+            pr, lat, el = run(mode, n2, arr2)
+            if not all(np.array_equal(a, b) for a, b in zip(pr, proofs[mode][:n2])):
+                raise RuntimeError(f"under-load {mode} proofs differ from the saturated run's")
+            under[mode] = {"proofs_per_s": n2 / el, "p50_ms": _pct(lat, 0.5) * 1e3, "p99_ms": _pct(lat, 0.99) * 1e3}
+            log(f"served under load (0.5x): {mode} p50 {under[mode]['p50_ms']:.1f} ms p99 {under[mode]['p99_ms']:.1f} ms")
     srv.close()
     if len(res) < 2:
         return {"requests": requests, "key_levels": levels, **res}
@@ -389,6 +403,8 @@ def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipeli
             "pipelines": pipelines, "key_levels": levels,
             "sequential": res["sequential"], "continuous": res["continuous"],
             "speedup_proofs_per_s": res["continuous"]["proofs_per_s"] / res["sequential"]["proofs_per_s"],
+            "under_load": {"requests": min(requests, under_load_requests), "rate": f"0.5x plain capacity = {0.5 * capacity:.1f}/s",
+                           **under} if under else None,
             "proofs_identical_across_modes": bool(same)}
 
 
