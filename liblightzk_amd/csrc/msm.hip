@@ -354,9 +354,17 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_coarse(MsmBatchDev b
         }
         __syncthreads();
         const uint32_t total = lstart[MSM_COARSE - 1] + cnt[MSM_COARSE - 1];
-        for (uint32_t p = tid; p < total; p += SC_THREADS) {
-            const uint32_t h = skey[p];
-            out[cur[h] + (p - lstart[h])] = stage[p];
+        if (d.tp == 2) {   // packed: point index < 2^24 and fine key < 128 share one word with the sign
+            uint32_t* out32 = (uint32_t*)out;
+            for (uint32_t p = tid; p < total; p += SC_THREADS) {
+                const uint32_t h = skey[p];
+                out32[cur[h] + (p - lstart[h])] = stage[p].x | (stage[p].y << 24);
+            }
+        } else {
+            for (uint32_t p = tid; p < total; p += SC_THREADS) {
+                const uint32_t h = skey[p];
+                out[cur[h] + (p - lstart[h])] = stage[p];
+            }
         }
         __syncthreads();
         if (tid < MSM_COARSE) cur[tid] += cnt[tid];
@@ -382,6 +390,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_fine(MsmBatchDev b, 
     if (tid < FB) cur[tid] = of[tid];
     const uint64_t ebase = space(b, SP_ENTRY)[ii];
     const uint2* in = coarse + ebase;
+    const uint32_t* in32 = (const uint32_t*)in;   // packed entries (tp == 2)
+    const bool packed = d.tp == 2;
     uint32_t* out = sorted + ebase;
     for (uint32_t base = g0; base < g1; base += SC_ROUND) {
         if (tid < FB) cnt[tid] = 0;
@@ -392,7 +402,12 @@ __global__ void __launch_bounds__(SC_THREADS) k_msm_scatter_fine(MsmBatchDev b, 
             const uint32_t e = base + tid + k * SC_THREADS;
             rank[k] = SC_NONE;
             if (e < g1) {
-                v[k] = in[e];
+                if (packed) {
+                    const uint32_t w = in32[e];
+                    v[k] = make_uint2(w & 0x80ffffffu, (w >> 24) & 0x7fu);
+                } else {
+                    v[k] = in[e];
+                }
                 rank[k] = atomicAdd(&cnt[v[k].y], 1u);
             }
         }

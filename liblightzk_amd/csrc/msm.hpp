@@ -70,7 +70,8 @@ struct MsmInst {
     // pipeline) serves Lg consecutive digit windows, so W = ceil(Wd / Lg) and
     // n = nd * Lg virtual points. Plain bases: nd = n, Wd = W, Lg = 1.
     uint32_t nd, Wd, Lg;
-    uint32_t tp;           // scatter in two passes (coarse groups, then buckets)
+    uint32_t tp;           // scatter in two passes (coarse groups, then buckets): 0 no, 1 yes
+                           // (8-byte coarse entries), 2 yes (packed 4-byte coarse entries)
     // Digit window j covers bits [j*wb + min(j, wx), +wb + (j < wx)): the 255
     // bits of a signed recoding of a scalar < 2^254 split as evenly as the W
     // windows allow (widths c and c - 1), so the top window is not a sliver

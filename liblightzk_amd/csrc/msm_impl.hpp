@@ -436,6 +436,9 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     for (size_t i = 0; i < count; ++i) {
         MsmInst& d = L.inst[i];
         d.tp = msm_two_pass(d.n, d.NB, batch_entries) ? 1u : 0u;
+        // coarse entries fit one word (index | fine key << 24 | sign << 31)
+        // when point indices stay below 2^24 and a group has at most 128 buckets
+        if (d.tp && d.n <= (1u << 24) && d.NB / MSM_COARSE <= 128) d.tp = 2;
         if (d.tp) L.two_pass = true;
         const size_t s = SP_CGROUP;
         L.begin[s * (count + 1) + i + 1] = L.begin[s * (count + 1) + i] + (d.tp ? (uint64_t)d.W * MSM_COARSE : 0);
