@@ -63,7 +63,15 @@ def test_proof_bit_exact_and_verifies(pair):
     assert not o.verify(z[: 4 * 9], bad)
 
 
-@pytest.mark.parametrize("p", [3, 6])
+def test_matmul_setup_rejects_bad_sizes(ctx):
+    import liblightzk_amd as z
+    for p in (0, 1, 256):
+        with pytest.raises(z.HzkError) as e:
+            z.Groth16.matmul(ctx, 4, p)
+        assert "rc=-1" in str(e.value) and "p" in str(e.value)
+
+
+@pytest.mark.parametrize("p", [2, 3, 6])
 def test_matmul_statement_bit_exact(ctx, orc, p):
     """The reference's MatMul statement (xpu-spmat.cuh:33-65): sizes, witness,
     key points and proofs equal the oracle's; the proof verifies."""

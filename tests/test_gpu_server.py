@@ -137,10 +137,11 @@ def test_injected_failure_is_isolated(oracles):
         assert s.stats()["failed"] == 1
 
 
-def test_server_restart_from_key_files_with_level_tables(ctx, oracles, tmp_path):
+def test_server_restart_from_key_files_with_level_tables(ctx, orc, oracles, tmp_path):
     """A server restarted from saved proving keys (hzk_server_add_circuit_file),
-    one of them proving over level tables (hzk_server_set_levels), serves a
-    mixed stream whose proofs equal the closed form."""
+    one of them proving over level tables (hzk_server_set_levels) and one the
+    reference's MatMul statement, serves a mixed stream whose proofs equal the
+    closed form."""
     import liblightzk_amd as z
     paths = {}
     for lg in LOGS:
@@ -148,14 +149,27 @@ def test_server_restart_from_key_files_with_level_tables(ctx, oracles, tmp_path)
         paths[lg] = str(tmp_path / f"c{lg}.hzk")
         g.save(paths[lg])
         g.close()
+    mm = z.Groth16.matmul(ctx, 6, 4)          # the reference's MatMul statement, p = 4
+    mm_path = str(tmp_path / "matmul.hzk")
+    mm.save(mm_path)
+    mm.close()
     s = z.ProofServer(0, max_batch=16, mode="continuous")
     cids = {lg: s.add_circuit_file(paths[lg]) for lg in LOGS}
     s.set_levels(cids[LOGS[-1]], 4)
+    mm_cid = s.add_circuit_file(mm_path)
     reqs = _requests(s, cids, 12, 9)
+    mm_reqs = [(s.witness(mm_cid, 70 + i), z.Groth16.rs(70 + i)) for i in range(3)]
     with s:
         s.start()
         ids = [s.submit(cids[lg], zz, rs) for lg, zz, rs in reqs]
+        mm_ids = [s.submit(mm_cid, zz, rs) for zz, rs in mm_reqs]
         for (lg, zz, rs), rid in zip(reqs, ids):
             st, proof, _ = s.wait(rid, 120)
             assert st == 1, s.last_error()
             assert np.array_equal(proof, oracles[lg].prove_trapdoor(zz, rs)), lg
+        o = orc.Groth16(6, matmul_p=4)
+        for i, ((zz, rs), rid) in enumerate(zip(mm_reqs, mm_ids)):
+            st, proof, _ = s.wait(rid, 120)
+            assert st == 1, s.last_error()
+            assert np.array_equal(zz, o.witness(70 + i))
+            assert np.array_equal(proof, o.prove_trapdoor(zz, rs))
