@@ -112,6 +112,8 @@ int stage_fence(hzk_ctx* ctx, int slot);
 void ntt_release(hzk_ctx* ctx);
 int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, const void* z, void* Az, void* Bz, void* Cz);
 int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale);
+// dst[k] = src[idx[k]], k < n, for elements of elem_bytes (a multiple of 16) on ctx's stream
+int gather_rows(hzk_ctx* ctx, const void* src, const uint32_t* idx, size_t n, size_t elem_bytes, void* dst);
 // in-place standard NTT (inverse: w^-1 and 1/n) and coset NTT over Fr (ntt.hip)
 int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 struct NttJob {

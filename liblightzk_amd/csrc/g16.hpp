@@ -33,6 +33,15 @@ struct hzk_g16 {
     std::vector<hzk::host::G1x> IC;   // verifying-key input commitments
     // buffers of the plain one-proof-at-a-time prover, and 1/(g^M - 1)
     void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *zinv = nullptr;
+    // The A and B query points of variables absent from A (resp. B) are the
+    // point at infinity (a third of them for the gate circuit, most of B for
+    // MatMul). The MSMs run over compacted copies holding only the finite
+    // points: idxA / idxB list their variables (at least one entry each), and
+    // a proof gathers z at those indices into its b / c buffers, which are
+    // free once h is computed and hold vec_len = max(M, nA, nB) elements.
+    uint32_t *idxA = nullptr, *idxB = nullptr;
+    size_t nA = 0, nB = 0, vec_len = 0;
+    void *A1c = nullptr, *B1c = nullptr, *B2c = nullptr;
     // fixed-base level tables of the key arrays for window size lc with lv
     // levels (lv <= 1: none); each MSM bucket set then serves lv windows
     void *A1L = nullptr, *B1L = nullptr, *B2L = nullptr, *L1L = nullptr, *H1L = nullptr;
@@ -63,5 +72,7 @@ void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof
 size_t g16_request_footprint(const hzk_g16* g);
 // allocate the plain prover's buffers (z, a, b, c) and upload 1/(g^M - 1)
 int g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g);
+// compacted A / B query arrays and their index lists from the full key arrays
+int g16_sparse_key(hzk_ctx* ctx, hzk_g16* g);
 
 }  // namespace hzk

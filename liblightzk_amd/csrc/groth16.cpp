@@ -210,8 +210,8 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
     const int c = level_window(N1, levels);
     const uint32_t lv = std::min<uint32_t>(levels, (uint32_t)((255 + c - 1) / c));
     struct T { void* src; size_t n; int group; void** dst; };
-    const T tabs[5] = {{g->A1, N1, 1, &g->A1L}, {g->B1, N1, 1, &g->B1L}, {g->L1, Ln, 1, &g->L1L}, {g->H1, Hn, 1, &g->H1L},
-                       {g->B2, N1, 2, &g->B2L}};
+    const T tabs[5] = {{g->A1c, g->nA, 1, &g->A1L}, {g->B1c, g->nB, 1, &g->B1L}, {g->L1, Ln, 1, &g->L1L}, {g->H1, Hn, 1, &g->H1L},
+                       {g->B2c, g->nB, 2, &g->B2L}};
     for (const T& t : tabs) {
         int rc = dev_alloc(ctx, g, t.n * lv * (t.group == 1 ? 64 : 128), t.dst);
         if (!rc) rc = hzk_bases_levels(ctx, t.group, c, (int)lv, t.src, t.n, *t.dst);
@@ -289,10 +289,45 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
         IC[i].canonical(ci);
         g->IC[i] = G.mul(ci);
     }
+    rc = g16_sparse_key(ctx, g);
+    if (rc) return rc;
     return g16_prover_buffers(ctx, g);
 }
 
 }  // namespace
+
+// Compacted A and B query arrays: the variables whose A1 (B1) point is not
+// the point at infinity (B_i(tau) = 0 makes B1_i and B2_i infinity alike),
+// found on the host from the key arrays, then gathered on the device.
+int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
+    const size_t N1 = g->num_vars;
+    std::vector<uint64_t> pts(N1 * 8);
+    std::vector<uint32_t> ia, ib;
+    for (int which = 0; which < 2; ++which) {
+        HZK_HIP(ctx, hipMemcpy(pts.data(), which ? g->B1 : g->A1, N1 * 64, hipMemcpyDeviceToHost));
+        std::vector<uint32_t>& idx = which ? ib : ia;
+        idx.reserve(N1);
+        for (size_t i = 0; i < N1; ++i) {
+            const uint64_t* p = &pts[8 * i];
+            if (p[0] | p[1] | p[2] | p[3] | p[4] | p[5] | p[6] | p[7]) idx.push_back((uint32_t)i);
+        }
+        if (idx.empty()) idx.push_back(0);   // an MSM needs one point: infinity contributes nothing
+    }
+    g->nA = ia.size();
+    g->nB = ib.size();
+    g->vec_len = std::max(g->M, std::max(g->nA, g->nB));
+    int rc = dev_upload(ctx, g, ia.data(), ia.size() * 4, (void**)&g->idxA);
+    if (!rc) rc = dev_upload(ctx, g, ib.data(), ib.size() * 4, (void**)&g->idxB);
+    if (!rc) rc = dev_alloc(ctx, g, g->nA * 64, &g->A1c);
+    if (!rc) rc = dev_alloc(ctx, g, g->nB * 64, &g->B1c);
+    if (!rc) rc = dev_alloc(ctx, g, g->nB * 128, &g->B2c);
+    if (!rc) rc = gather_rows(ctx, g->A1, g->idxA, g->nA, 64, g->A1c);
+    if (!rc) rc = gather_rows(ctx, g->B1, g->idxB, g->nB, 64, g->B1c);
+    if (!rc) rc = gather_rows(ctx, g->B2, g->idxB, g->nB, 128, g->B2c);
+    if (rc) return rc;
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HZK_OK;
+}
 
 // Buffers of the plain prover (z and the three evaluation vectors) and
 // 1/(g^M - 1) for the coset division.
@@ -300,8 +335,8 @@ int hzk::g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g) {
     const size_t M = g->M;
     int rc = dev_alloc(ctx, g, g->num_vars * 32, &g->z);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->a);
-    if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->b);
-    if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->c);
+    if (!rc) rc = dev_alloc(ctx, g, g->vec_len * 32, &g->b);
+    if (!rc) rc = dev_alloc(ctx, g, g->vec_len * 32, &g->c);
     if (rc) return rc;
     Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
     return dev_upload(ctx, g, zi.v, 32, &g->zinv);
@@ -340,19 +375,25 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
         hzk_g16* g = reqs[i].g;
         const size_t N1 = g->num_vars, l = g->num_public;
         char* r = reqs[i].res;
+        // z at the variables of the finite A / B query points, into b / c
+        // (free once the H stage is done; vec_len elements each)
+        void *zA = reqs[i].b, *zB = reqs[i].c;
+        int rc = gather_rows(ctx, reqs[i].z, g->idxA, g->nA, 32, zA);
+        if (!rc) rc = gather_rows(ctx, reqs[i].z, g->idxB, g->nB, 32, zB);
+        if (rc) return rc;
         if (g->lv > 1) {
             const uint32_t c = (uint32_t)g->lc, L = g->lv;
-            j1.push_back({reqs[i].z, g->A1L, N1, r, c, L});
-            j1.push_back({reqs[i].z, g->B1L, N1, r + 96, c, L});
+            j1.push_back({zA, g->A1L, g->nA, r, c, L});
+            j1.push_back({zB, g->B1L, g->nB, r + 96, c, L});
             j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1L, N1 - l - 1, r + 384, c, L});
             j1.push_back({reqs[i].a, g->H1L, g->M - 1, r + 480, c, L});
-            j2.push_back({reqs[i].z, g->B2L, N1, r + 192, c, L});
+            j2.push_back({zB, g->B2L, g->nB, r + 192, c, L});
         } else {
-            j1.push_back({reqs[i].z, g->A1, N1, r, 0, 0});
-            j1.push_back({reqs[i].z, g->B1, N1, r + 96, 0, 0});
+            j1.push_back({zA, g->A1c, g->nA, r, 0, 0});
+            j1.push_back({zB, g->B1c, g->nB, r + 96, 0, 0});
             j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1, N1 - l - 1, r + 384, 0, 0});
             j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0, 0});
-            j2.push_back({reqs[i].z, g->B2, N1, r + 192, 0, 0});
+            j2.push_back({zB, g->B2c, g->nB, r + 192, 0, 0});
         }
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with
@@ -401,7 +442,7 @@ static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
 
 size_t g16_request_footprint(const hzk_g16* g) {
     // z + three evaluation vectors, and the scratch of its five MSMs with 25% slack
-    size_t bufs = (g->num_vars + 3 * g->M) * 32;
+    size_t bufs = (g->num_vars + g->M + 2 * g->vec_len) * 32;
     size_t N1 = g->num_vars, L = N1 - g->num_public - 1;
     size_t msm = 2 * msm_scratch_bound(N1, 128) + msm_scratch_bound(L, 128) + msm_scratch_bound(g->M - 1, 128) +
                  msm_scratch_bound(N1, 256);

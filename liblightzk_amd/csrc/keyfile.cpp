@@ -282,6 +282,7 @@ int hzk_g16_load(hzk_ctx* ctx, const char* path, hzk_g16** out) {
             }
         }
     }
+    if (!rc) rc = g16_sparse_key(ctx, g);
     if (!rc) rc = g16_prover_buffers(ctx, g);
     if (rc) {
         hzk_g16_free(g);

@@ -102,7 +102,9 @@ struct hzk_server {
 
 namespace {
 
-size_t bufset_bytes(const hzk_g16* g) { return (g->num_vars + 3 * g->M) * 32; }
+// z, a (M elements), b and c (vec_len elements: the H stage's B z and C z,
+// then the gathered scalars of the compacted A / B query MSMs)
+size_t bufset_bytes(const hzk_g16* g) { return (g->num_vars + g->M + 2 * g->vec_len) * 32; }
 
 void release(BufSet& b) {
     for (void* p : {b.z, b.a, b.b, b.c})
@@ -142,9 +144,9 @@ int alloc_request(hzk_server* s, Request* r) {
             if (!victim.z) break;
             release(victim);
         }
-        size_t zb = g->num_vars * 32, vb = g->M * 32;
-        if (hipMalloc(&b.z, zb) != hipSuccess || hipMalloc(&b.a, vb) != hipSuccess || hipMalloc(&b.b, vb) != hipSuccess ||
-            hipMalloc(&b.c, vb) != hipSuccess) {
+        size_t zb = g->num_vars * 32, vb = g->M * 32, wb = g->vec_len * 32;
+        if (hipMalloc(&b.z, zb) != hipSuccess || hipMalloc(&b.a, vb) != hipSuccess || hipMalloc(&b.b, wb) != hipSuccess ||
+            hipMalloc(&b.c, wb) != hipSuccess) {
             (void)hipGetLastError();
             release(b);
             return HZK_ERR_NOMEM;

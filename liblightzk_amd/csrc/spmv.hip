@@ -64,6 +64,30 @@ int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, co
     return HZK_OK;
 }
 
+// dst[k] = src[idx[k]] for elements of q 16-byte words: thread per word, so
+// the writes are coalesced and each element's words are read together
+__global__ void __launch_bounds__(256) k_gather_rows(const uint4* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                                                     uint32_t q, uint4* __restrict__ dst) {
+    const uint64_t total = n * q;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = g / q;
+        dst[g] = src[(uint64_t)idx[e] * q + (g - e * q)];
+    }
+}
+
+int gather_rows(hzk_ctx* ctx, const void* src, const uint32_t* idx, size_t n, size_t elem_bytes, void* dst) {
+    HZK_CHECK(ctx, src && idx && dst && elem_bytes % 16 == 0 && elem_bytes > 0, HZK_ERR_INVALID, "gather_rows: bad argument");
+    if (n == 0) return HZK_OK;
+    const uint32_t q = (uint32_t)(elem_bytes / 16);
+    const uint64_t words = (uint64_t)n * q;
+    const uint64_t blocks = (words + 255) / 256;
+    const unsigned grid = (unsigned)(blocks < (uint64_t)ctx->num_cus * 32 ? blocks : (uint64_t)ctx->num_cus * 32);
+    TimedLaunch tl(ctx, "gather_rows");
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, ctx->stream, (const uint4*)src, idx, (uint64_t)n, q, (uint4*)dst);
+    HZK_HIP(ctx, hipGetLastError());
+    return HZK_OK;
+}
+
 int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale) {
     HZK_CHECK(ctx, A && B && C, HZK_ERR_INVALID, "pointwise_abc: null argument");
     if (n == 0) return HZK_OK;
