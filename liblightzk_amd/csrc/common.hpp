@@ -114,6 +114,8 @@ int spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C, co
 int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C, const void* scale);
 // dst[k] = src[idx[k]], k < n, for elements of elem_bytes (a multiple of 16) on ctx's stream
 int gather_rows(hzk_ctx* ctx, const void* src, const uint32_t* idx, size_t n, size_t elem_bytes, void* dst);
+// dst[k] = z[idx[k]] * r over Fr (Montgomery; r: 32 bytes on the host), k < n
+int gather_scale_fr(hzk_ctx* ctx, const void* z, const uint32_t* idx, size_t n, const void* r_host, void* dst);
 // in-place standard NTT (inverse: w^-1 and 1/n) and coset NTT over Fr (ntt.hip)
 int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 struct NttJob {

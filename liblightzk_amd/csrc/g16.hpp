@@ -36,15 +36,19 @@ struct hzk_g16 {
     // The A and B query points of variables absent from A (resp. B) are the
     // point at infinity (a third of them for the gate circuit, most of B for
     // MatMul). The MSMs run over compacted copies holding only the finite
-    // points: idxA / idxB list their variables (at least one entry each), and
-    // a proof gathers z at those indices into its b / c buffers, which are
-    // free once h is computed and hold vec_len = max(M, nA, nB) elements.
+    // points: idxA / idxB list their variables (at least one entry each).
+    // B1 enters the proof only through C = sum L + sum H + s A + r B1 - r s d1
+    //   = [sum_{i>l} z_i L_i + sum_j h_j H_j + sum_{i in B} (r z_i) B1_i] + s A + r beta1,
+    // so L, H and the compacted B1 form one MSM over C1 = L1 | H1 | B1c
+    // (nC = nL + M - 1 + nB points). A proof's b buffer (free once h is
+    // computed) takes z_A | z_B (vec_b elements), its c buffer
+    // z_{l+1..} | h | r z_B (vec_c elements).
     uint32_t *idxA = nullptr, *idxB = nullptr;
-    size_t nA = 0, nB = 0, vec_len = 0;
-    void *A1c = nullptr, *B1c = nullptr, *B2c = nullptr;
+    size_t nA = 0, nB = 0, nC = 0, vec_b = 0, vec_c = 0;
+    void *A1c = nullptr, *B2c = nullptr, *C1 = nullptr;
     // fixed-base level tables of the key arrays for window size lc with lv
     // levels (lv <= 1: none); each MSM bucket set then serves lv windows
-    void *A1L = nullptr, *B1L = nullptr, *B2L = nullptr, *L1L = nullptr, *H1L = nullptr;
+    void *A1L = nullptr, *C1L = nullptr, *B2L = nullptr;
     int lc = 0;
     uint32_t lv = 1;
     std::vector<void*> owned;
@@ -58,13 +62,14 @@ namespace hzk {
 struct G16Req {
     hzk_g16* g;
     void *z, *a, *b, *c;
-    char res[576];   // A1, B1 (G1, 96 B each), B2 (G2, 192 B), L, H (G1)
+    const void* rs;  // r | s (host, Montgomery)
+    char res[384];   // sum A (G1, 96 B), the merged C MSM (G1, 96 B), sum B2 (G2, 192 B)
 };
 // SpMV and the H pipeline (3 iNTT, 3 coset NTT, (AB - C)/Z, coset iNTT) for
 // several requests, enqueued on ctx's stream with batched transforms.
 int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n);
-// the five MSMs of several requests as two batched MSMs (G1: 4 per request,
-// G2: 1 per request); results land in reqs[i].res on the host
+// the MSMs of several requests as two batched MSMs (G1: A and the merged C
+// MSM per request, G2: B per request); results land in reqs[i].res on the host
 int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph);
 // phase-6 assembly on the host; proof = A | B | C (affine, 256 B)
 void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof);

@@ -191,11 +191,12 @@ int level_window(size_t n, uint32_t levels) {
     return best;
 }
 
-// Level tables of the five key arrays with `levels` levels each (point
-// t*n + i = 2^(c t) P_i): a proof's MSMs then reduce ceil(W / levels) bucket
-// sets each instead of W. levels <= 1 frees them (plain key arrays).
+// Level tables of the three MSM base arrays (compacted A1, the merged C1,
+// compacted B2) with `levels` levels each (point t*n + i = 2^(c t) P_i): a
+// proof's MSMs then reduce ceil(W / levels) bucket sets each instead of W.
+// levels <= 1 frees them (plain arrays).
 int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
-    for (void** p : {&g->A1L, &g->B1L, &g->L1L, &g->H1L, &g->B2L}) {
+    for (void** p : {&g->A1L, &g->C1L, &g->B2L}) {
         if (*p) {
             (void)hipStreamSynchronize(ctx->stream);
             (void)hipFree(*p);
@@ -206,12 +207,10 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
     g->lv = 1;
     g->lc = 0;
     if (levels <= 1) return 0;
-    const size_t N1 = g->num_vars, Ln = N1 - g->num_public - 1, Hn = g->M - 1;
-    const int c = level_window(N1, levels);
+    const int c = level_window(g->num_vars, levels);
     const uint32_t lv = std::min<uint32_t>(levels, (uint32_t)((255 + c - 1) / c));
     struct T { void* src; size_t n; int group; void** dst; };
-    const T tabs[5] = {{g->A1c, g->nA, 1, &g->A1L}, {g->B1c, g->nB, 1, &g->B1L}, {g->L1, Ln, 1, &g->L1L}, {g->H1, Hn, 1, &g->H1L},
-                       {g->B2c, g->nB, 2, &g->B2L}};
+    const T tabs[3] = {{g->A1c, g->nA, 1, &g->A1L}, {g->C1, g->nC, 1, &g->C1L}, {g->B2c, g->nB, 2, &g->B2L}};
     for (const T& t : tabs) {
         int rc = dev_alloc(ctx, g, t.n * lv * (t.group == 1 ? 64 : 128), t.dst);
         if (!rc) rc = hzk_bases_levels(ctx, t.group, c, (int)lv, t.src, t.n, *t.dst);
@@ -298,7 +297,8 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
 
 // Compacted A and B query arrays: the variables whose A1 (B1) point is not
 // the point at infinity (B_i(tau) = 0 makes B1_i and B2_i infinity alike),
-// found on the host from the key arrays, then gathered on the device.
+// found on the host from the key arrays, then gathered on the device; and
+// the merged base array C1 = L1 | H1 | B1c of the C MSM.
 int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
     const size_t N1 = g->num_vars;
     std::vector<uint64_t> pts(N1 * 8);
@@ -313,18 +313,24 @@ int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
         }
         if (idx.empty()) idx.push_back(0);   // an MSM needs one point: infinity contributes nothing
     }
+    const size_t nL = N1 - g->num_public - 1, nH = g->M - 1;
     g->nA = ia.size();
     g->nB = ib.size();
-    g->vec_len = std::max(g->M, std::max(g->nA, g->nB));
+    g->nC = nL + nH + g->nB;
+    g->vec_b = std::max(g->M, g->nA + g->nB);
+    g->vec_c = std::max(g->M, g->nC);
     int rc = dev_upload(ctx, g, ia.data(), ia.size() * 4, (void**)&g->idxA);
     if (!rc) rc = dev_upload(ctx, g, ib.data(), ib.size() * 4, (void**)&g->idxB);
     if (!rc) rc = dev_alloc(ctx, g, g->nA * 64, &g->A1c);
-    if (!rc) rc = dev_alloc(ctx, g, g->nB * 64, &g->B1c);
     if (!rc) rc = dev_alloc(ctx, g, g->nB * 128, &g->B2c);
+    if (!rc) rc = dev_alloc(ctx, g, g->nC * 64, &g->C1);
     if (!rc) rc = gather_rows(ctx, g->A1, g->idxA, g->nA, 64, g->A1c);
-    if (!rc) rc = gather_rows(ctx, g->B1, g->idxB, g->nB, 64, g->B1c);
     if (!rc) rc = gather_rows(ctx, g->B2, g->idxB, g->nB, 128, g->B2c);
+    // C1 = L1 | H1 | B1 at idxB
+    if (!rc) rc = gather_rows(ctx, g->B1, g->idxB, g->nB, 64, (char*)g->C1 + (nL + nH) * 64);
     if (rc) return rc;
+    HZK_HIP(ctx, hipMemcpyAsync(g->C1, g->L1, nL * 64, hipMemcpyDeviceToDevice, ctx->stream));
+    HZK_HIP(ctx, hipMemcpyAsync((char*)g->C1 + nL * 64, g->H1, nH * 64, hipMemcpyDeviceToDevice, ctx->stream));
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HZK_OK;
 }
@@ -335,8 +341,8 @@ int hzk::g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g) {
     const size_t M = g->M;
     int rc = dev_alloc(ctx, g, g->num_vars * 32, &g->z);
     if (!rc) rc = dev_alloc(ctx, g, M * 32, &g->a);
-    if (!rc) rc = dev_alloc(ctx, g, g->vec_len * 32, &g->b);
-    if (!rc) rc = dev_alloc(ctx, g, g->vec_len * 32, &g->c);
+    if (!rc) rc = dev_alloc(ctx, g, g->vec_b * 32, &g->b);
+    if (!rc) rc = dev_alloc(ctx, g, g->vec_c * 32, &g->c);
     if (rc) return rc;
     Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
     return dev_upload(ctx, g, zi.v, 32, &g->zinv);
@@ -373,28 +379,22 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
     std::vector<MsmJob> j1, j2;
     for (size_t i = 0; i < n; ++i) {
         hzk_g16* g = reqs[i].g;
-        const size_t N1 = g->num_vars, l = g->num_public;
+        const size_t N1 = g->num_vars, l = g->num_public, nL = N1 - l - 1, nH = g->M - 1;
         char* r = reqs[i].res;
-        // z at the variables of the finite A / B query points, into b / c
-        // (free once the H stage is done; vec_len elements each)
-        void *zA = reqs[i].b, *zB = reqs[i].c;
-        int rc = gather_rows(ctx, reqs[i].z, g->idxA, g->nA, 32, zA);
-        if (!rc) rc = gather_rows(ctx, reqs[i].z, g->idxB, g->nB, 32, zB);
+        // scalars into the buffers the H stage no longer needs:
+        // b = z_A | z_B, c = z_{l+1..} | h | r z_B
+        char* b = (char*)reqs[i].b;
+        char* c = (char*)reqs[i].c;
+        int rc = gather_rows(ctx, reqs[i].z, g->idxA, g->nA, 32, b);
+        if (!rc) rc = gather_rows(ctx, reqs[i].z, g->idxB, g->nB, 32, b + g->nA * 32);
+        if (!rc) rc = gather_scale_fr(ctx, reqs[i].z, g->idxB, g->nB, reqs[i].rs, c + (nL + nH) * 32);
         if (rc) return rc;
-        if (g->lv > 1) {
-            const uint32_t c = (uint32_t)g->lc, L = g->lv;
-            j1.push_back({zA, g->A1L, g->nA, r, c, L});
-            j1.push_back({zB, g->B1L, g->nB, r + 96, c, L});
-            j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1L, N1 - l - 1, r + 384, c, L});
-            j1.push_back({reqs[i].a, g->H1L, g->M - 1, r + 480, c, L});
-            j2.push_back({zB, g->B2L, g->nB, r + 192, c, L});
-        } else {
-            j1.push_back({zA, g->A1c, g->nA, r, 0, 0});
-            j1.push_back({zB, g->B1c, g->nB, r + 96, 0, 0});
-            j1.push_back({(char*)reqs[i].z + (l + 1) * 32, g->L1, N1 - l - 1, r + 384, 0, 0});
-            j1.push_back({reqs[i].a, g->H1, g->M - 1, r + 480, 0, 0});
-            j2.push_back({zB, g->B2c, g->nB, r + 192, 0, 0});
-        }
+        HZK_HIP(ctx, hipMemcpyAsync(c, (char*)reqs[i].z + (l + 1) * 32, nL * 32, hipMemcpyDeviceToDevice, ctx->stream));
+        HZK_HIP(ctx, hipMemcpyAsync(c + nL * 32, reqs[i].a, nH * 32, hipMemcpyDeviceToDevice, ctx->stream));
+        const uint32_t wc = g->lv > 1 ? (uint32_t)g->lc : 0, L = g->lv > 1 ? g->lv : 0;
+        j1.push_back({b, g->lv > 1 ? g->A1L : g->A1c, g->nA, r, wc, L});
+        j1.push_back({c, g->lv > 1 ? g->C1L : g->C1, g->nC, r + 96, wc, L});
+        j2.push_back({b + g->nA * 32, g->lv > 1 ? g->B2L : g->B2c, g->nB, r + 192, wc, L});
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with
     // the G1 batch: the latency-bound tails of one (bucket reduction, window
@@ -442,14 +442,17 @@ static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
 
 size_t g16_request_footprint(const hzk_g16* g) {
     // z + three evaluation vectors, and the scratch of its five MSMs with 25% slack
-    size_t bufs = (g->num_vars + g->M + 2 * g->vec_len) * 32;
-    size_t N1 = g->num_vars, L = N1 - g->num_public - 1;
-    size_t msm = 2 * msm_scratch_bound(N1, 128) + msm_scratch_bound(L, 128) + msm_scratch_bound(g->M - 1, 128) +
-                 msm_scratch_bound(N1, 256);
+    size_t bufs = (g->num_vars + g->M + g->vec_b + g->vec_c) * 32;
+    size_t msm = msm_scratch_bound(g->nA, 128) + msm_scratch_bound(g->nC, 128) + msm_scratch_bound(g->nB, 256);
     return bufs + msm + msm / 4;
 }
 
 // Phase 6 on the host; proof = A (64 B) | B (128 B) | C (64 B), affine.
+// With the merged C MSM (sum L + sum H + sum (r z_i) B1_i):
+//   A = alpha1 + sum A + r delta1, B = beta2 + sum B2 + s delta2,
+//   C = merged + s A + r beta1
+// (= sum L + sum H + s A + r B1 - r s delta1 with B1 = beta1 + sum B1 + s delta1,
+// groth16.cuh:210-215).
 void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof) {
     Fr r = Fr::load(rs), s = Fr::load((const char*)rs + 32);
     auto jac1 = [](const char* p) {   // normalised Jacobian (x, y, 1) or (0, 1, 0)
@@ -462,16 +465,14 @@ void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof
         if (z.is_zero()) return G2x::inf();
         return G2x::from_affine(host::Fq2::load(p), host::Fq2::load(p + 64));
     };
-    G1x sA = jac1(res), sB1 = jac1(res + 96), sL = jac1(res + 384), sH = jac1(res + 480);
+    G1x sA = jac1(res), merged = jac1(res + 96);
     G2x sB2 = jac2(res + 192);
-    uint64_t rc[4], sc[4], rsc[4];
+    uint64_t rc[4], sc[4];
     r.canonical(rc);
     s.canonical(sc);
-    (r * s).canonical(rsc);
     G1x A = g->alpha1.add(sA).add(g->delta1.mul(rc));
     G2x B = g->beta2.add(sB2).add(g->delta2.mul(sc));
-    G1x B1 = g->beta1.add(sB1).add(g->delta1.mul(sc));
-    G1x C = sL.add(sH).add(A.mul(sc)).add(B1.mul(rc)).add(g->delta1.mul(rsc).neg());
+    G1x C = merged.add(A.mul(sc)).add(g->beta1.mul(rc));
     char* out = (char*)proof;
     A.store_affine(out);
     B.store_affine(out + 64);
@@ -579,7 +580,7 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
     if (!ctx || !g) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, z_host && rs_host && proof_host, HZK_ERR_INVALID, "g16_prove: null argument");
     HZK_HIP(ctx, hipMemcpyAsync(g->z, z_host, g->num_vars * 32, hipMemcpyHostToDevice, ctx->stream));
-    G16Req r{g, g->z, g->a, g->b, g->c, {}};
+    G16Req r{g, g->z, g->a, g->b, g->c, rs_host, {}};
     int rc = g16_h_batch(ctx, &r, 1);
     if (!rc) rc = g16_msm_batch(ctx, &r, 1, 0);
     if (rc) return rc;

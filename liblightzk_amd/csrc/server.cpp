@@ -102,9 +102,9 @@ struct hzk_server {
 
 namespace {
 
-// z, a (M elements), b and c (vec_len elements: the H stage's B z and C z,
-// then the gathered scalars of the compacted A / B query MSMs)
-size_t bufset_bytes(const hzk_g16* g) { return (g->num_vars + g->M + 2 * g->vec_len) * 32; }
+// z, a (M elements), b and c (vec_b / vec_c elements: the H stage's B z and
+// C z, then the scalars of the A, merged C and B2 MSMs)
+size_t bufset_bytes(const hzk_g16* g) { return (g->num_vars + g->M + g->vec_b + g->vec_c) * 32; }
 
 void release(BufSet& b) {
     for (void* p : {b.z, b.a, b.b, b.c})
@@ -144,9 +144,9 @@ int alloc_request(hzk_server* s, Request* r) {
             if (!victim.z) break;
             release(victim);
         }
-        size_t zb = g->num_vars * 32, vb = g->M * 32, wb = g->vec_len * 32;
-        if (hipMalloc(&b.z, zb) != hipSuccess || hipMalloc(&b.a, vb) != hipSuccess || hipMalloc(&b.b, wb) != hipSuccess ||
-            hipMalloc(&b.c, wb) != hipSuccess) {
+        size_t zb = g->num_vars * 32, vb = g->M * 32;
+        if (hipMalloc(&b.z, zb) != hipSuccess || hipMalloc(&b.a, vb) != hipSuccess ||
+            hipMalloc(&b.b, g->vec_b * 32) != hipSuccess || hipMalloc(&b.c, g->vec_c * 32) != hipSuccess) {
             (void)hipGetLastError();
             release(b);
             return HZK_ERR_NOMEM;
@@ -241,7 +241,7 @@ int stage_h(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     for (Request* r : rs) {
         const void* src = r->zp ? r->zp : (const void*)r->z.data();
         HZK_HIP(ctx, hipMemcpyAsync(r->dz, src, r->zbytes, hipMemcpyHostToDevice, ctx->stream));
-        reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
+        reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, r->rs, {}});
     }
     return g16_h_batch(ctx, reqs.data(), reqs.size());
 }
@@ -250,7 +250,7 @@ int stage_h(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
 int stage_msm(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     if (rs.empty()) return 0;
     std::vector<G16Req> reqs;
-    for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, {}});
+    for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, r->rs, {}});
     int rc = g16_msm_batch(ctx, reqs.data(), reqs.size(), 0);
     if (rc) return rc;
 #pragma omp parallel for schedule(dynamic, 1)

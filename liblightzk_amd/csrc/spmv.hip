@@ -10,6 +10,7 @@
 // One lane per row over the three matrices in one launch (synthetic R1CS rows
 // hold 1-3 nonzeros, so a row is a handful of gathers and Fr products; SpMV is
 // gather-bound, ~68 B per nonzero).
+#include <cstring>
 #include "common.hpp"
 #include "field.hpp"
 
@@ -84,6 +85,26 @@ int gather_rows(hzk_ctx* ctx, const void* src, const uint32_t* idx, size_t n, si
     const unsigned grid = (unsigned)(blocks < (uint64_t)ctx->num_cus * 32 ? blocks : (uint64_t)ctx->num_cus * 32);
     TimedLaunch tl(ctx, "gather_rows");
     hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, ctx->stream, (const uint4*)src, idx, (uint64_t)n, q, (uint4*)dst);
+    HZK_HIP(ctx, hipGetLastError());
+    return HZK_OK;
+}
+
+// dst[k] = z[idx[k]] * r (Montgomery product)
+__global__ void __launch_bounds__(256) k_gather_scale(const fr_t* __restrict__ z, const uint32_t* __restrict__ idx, size_t n,
+                                                      fr_t r, fr_t* __restrict__ dst) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = z[idx[k]] * r;
+}
+
+int gather_scale_fr(hzk_ctx* ctx, const void* z, const uint32_t* idx, size_t n, const void* r_host, void* dst) {
+    HZK_CHECK(ctx, z && idx && r_host && dst, HZK_ERR_INVALID, "gather_scale_fr: null argument");
+    if (n == 0) return HZK_OK;
+    fr_t r;
+    std::memcpy(r.l, r_host, 32);
+    const uint64_t blocks = (n + 255) / 256;
+    const unsigned grid = (unsigned)(blocks < (uint64_t)ctx->num_cus * 16 ? blocks : (uint64_t)ctx->num_cus * 16);
+    TimedLaunch tl(ctx, "gather_rows");
+    hipLaunchKernelGGL(k_gather_scale, dim3(grid), dim3(256), 0, ctx->stream, (const fr_t*)z, idx, n, r, (fr_t*)dst);
     HZK_HIP(ctx, hipGetLastError());
     return HZK_OK;
 }
