@@ -147,25 +147,28 @@ def groth16_reference_presets(ctx, ps=(39, 63, 100), proofs=5):
 
 def served_msm_batches(ctx, reps=3, seed=4):
     """The MSM stage of 32 served requests of the 40/30/20/10 mix (13 x 2^12,
-    10 x 2^16, 6 x 2^18, 3 x 2^20 constraints) launched alone: one G1 batch of
-    4 MSMs per request (128 MSMs) and one G2 batch of 1 MSM per request (32
-    MSMs), sizes 2^k, seeded uniform scalars and bases k_i*G (wide-batch plans)."""
+    10 x 2^16, 6 x 2^18, 3 x 2^20 constraints, n = 2^k) launched alone, with
+    the prover's MSM shapes: one G1 batch of 2 MSMs per request (the compacted
+    A query, 2n/3 points, and the merged C MSM, 8n/3 points: L, H and the
+    compacted B1) and one G2 batch of 1 MSM per request (the compacted B2
+    query, 2n/3 points); seeded uniform scalars and bases k_i*G."""
     import liblightzk_amd as z
     mix = [12] * 13 + [16] * 10 + [18] * 6 + [20] * 3
-    nmax = 1 << 20
+    nmax = 3 << 20                       # the merged C MSM of a 2^20 request: 8/3 * 2^20 points
     s = z.gen_field(seed, 0, 2 * nmax, "fr")
     k = z.gen_field(seed, 1, nmax, "fr")
     ds, dk = ctx.upload(s), ctx.upload(k)
-    b1, b2 = ctx.alloc(64 * nmax), ctx.alloc(128 * nmax)
+    b1, b2 = ctx.alloc(64 * nmax), ctx.alloc(128 * (1 << 20))
     ctx.fixbase_g1(dk, nmax, b1)
-    ctx.fixbase_g2(dk, nmax, b2)
-    out1, out2 = ctx.alloc(96 * 4 * len(mix)), ctx.alloc(192 * len(mix))
+    ctx.fixbase_g2(dk, 1 << 20, b2)
+    out1, out2 = ctx.alloc(96 * 2 * len(mix)), ctx.alloc(192 * len(mix))
     rng = np.random.default_rng([seed, 9])
     j1, j2 = [], []
     for i, lg in enumerate(mix):
-        for t in range(4):
-            j1.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b1.ptr, 1 << lg, out1.ptr + 96 * (4 * i + t)))
-        j2.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b2.ptr, 1 << lg, out2.ptr + 192 * i))
+        n = 1 << lg
+        for t, m in enumerate((2 * n // 3, 8 * n // 3)):
+            j1.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b1.ptr, m, out1.ptr + 96 * (2 * i + t)))
+        j2.append((ds.ptr + 32 * int(rng.integers(0, nmax)), b2.ptr, 2 * n // 3, out2.ptr + 192 * i))
     res = {}
     for grp, jobs in ((1, j1), (2, j2)):
         ctx.msm_batch(grp, jobs)   # warm-up (scratch, plans)
