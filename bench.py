@@ -425,7 +425,8 @@ def main():
                     help="fixed-base level tables per proving key (levels per bucket set; 1 = plain key arrays)")
     ap.add_argument("--serve-logs", default="12,16,18,20")
     ap.add_argument("--serve-weights", default="40,30,20,10")
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="length of the served request stream (a finite Poisson stream of this many requests)")
     ap.add_argument("--serve-rate-mult", type=float, default=4.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--serve-pipelines", type=int, default=2)
