@@ -21,6 +21,9 @@
 //            A = alpha1 + sum z_i A_i + r delta1
 //            B = beta2 + sum z_i B2_i + s delta2,  B1 = beta1 + sum z_i B1_i + s delta1
 //            C = sum_{i>l} z_i L_i + sum_j h_j H_j + s A + r B1 - r s delta1
+//              = [sum_{i>l} z_i L_i + sum_j h_j H_j + sum_i (r z_i) B1_i] + s A + r beta1
+//            The MSMs skip the key's points at infinity (compacted A1, B1,
+//            B2) and the bracket is one merged MSM; see g16.hpp.
 // Streams: 2 circuit structure, 3 witness inputs, 4 toxic waste
 // (tau, alpha, beta, gamma, delta), 5 prover randomness (r, s).
 #include <omp.h>
@@ -441,7 +444,7 @@ static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
 }
 
 size_t g16_request_footprint(const hzk_g16* g) {
-    // z + three evaluation vectors, and the scratch of its five MSMs with 25% slack
+    // z, a, b, c and the scratch of its three MSMs (A, merged C, B2) with 25% slack
     size_t bufs = (g->num_vars + g->M + g->vec_b + g->vec_c) * 32;
     size_t msm = msm_scratch_bound(g->nA, 128) + msm_scratch_bound(g->nC, 128) + msm_scratch_bound(g->nB, 256);
     return bufs + msm + msm / 4;
