@@ -119,6 +119,12 @@ int hzk_msm_last_stats(hzk_ctx* ctx, uint32_t* out5);
  * table of the primitive deg-th root (inverse = 0) or its inverse (inverse = 1,
  * then scaled by 1/deg unless factor is given). factor: device Fr or NULL. */
 int hzk_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, const void* omega_powers, int inverse, const void* factor);
+/* The reference's ntt_setup table for an M-point domain, written to host
+ * memory as 6M + 2 Fr elements (replaces ntt_setup<FieldT, ntt_config*>,
+ * include/protocol/groth16-operator.cuh:6-32): [0, M) w_M^i, [M, 2M) w_M^-i,
+ * 2M: M^-1, [2M+1, 4M+1) w_2M^i, [4M+1, 6M+1) w_2M^-i, 6M+1: (4M)^-1 (the
+ * value the reference writes there). M a power of two <= 2^26. Needs no GPU. */
+int hzk_ntt_setup(size_t M, void* host_out);
 /* coset NTT on g*<w>, g = 5 (alt_bn128 Fr multiplicative generator,
  * alt_bn128_init.cu:50): inverse = 0 evaluates the coefficients at g*w^k;
  * inverse = 1 interpolates from those evaluations back to coefficients.
@@ -188,7 +194,8 @@ int hzk_gen_field(uint64_t seed, uint64_t stream, size_t n, int kind, void* host
 typedef struct hzk_g16 hzk_g16;
 int hzk_g16_setup(hzk_ctx* ctx, uint64_t seed, int log_m, int num_public, hzk_g16** out);
 void hzk_g16_free(hzk_g16* g);
-/* {M, num_vars, num_public, num_gates, nnz A, nnz B, nnz C, proving-key bytes} */
+/* {M, num_vars, num_public, num_gates, nnz A, nnz B, nnz C, resident device bytes of the key: CSR,
+ * full and compacted query arrays, level tables and the plain prover's buffers} */
 int hzk_g16_info(const hzk_g16* g, uint64_t* out8);
 /* seeded satisfying witness z[num_vars] (host, Fr Montgomery; z[0] = 1) */
 int hzk_g16_witness(const hzk_g16* g, uint64_t wseed, void* z_host);
@@ -226,7 +233,8 @@ int hzk_g16_load(hzk_ctx* ctx, const char* path, hzk_g16** out);
  * admitted requests on one stream while the batched MSMs of the previous
  * step's requests run on another; mode 1: one request at a time, no overlap.
  * Admission is in arrival order while the requests' device footprint fits
- * mem_budget (0 = half the free memory); a request that can never fit fails
+ * mem_budget (0 = half the device memory still free when hzk_server_start
+ * runs, after the circuits' keys are resident); a request that can never fit fails
  * with HZK_ERR_NOMEM. A failing stage fails only its requests. A proof is a
  * function of (circuit, z, r|s) alone: the same alone or in any mix. */
 typedef struct hzk_server hzk_server;

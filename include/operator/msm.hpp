@@ -36,7 +36,7 @@ void multi_scalar_multiplication_g2(size_t length, devFdT* scalars, dG2* points,
 // include/operator/msm.cuh:5-24: dest[i] = k_i * G1 as reference records
 // (Z = 1). The reference converts the scalars out of Montgomery form in
 // place; these are left unchanged.
-template <typename dG1, typename devFdT>
+template <typename devFdT, typename dG1>
 void fix_base_multi_scalar_multiplication_g1(dG1* dest, devFdT* scalars, size_t length) {
     lightzk::run([&](hzk_ctx* c) {
         void* tmp = nullptr;
@@ -51,7 +51,7 @@ void fix_base_multi_scalar_multiplication_g1(dG1* dest, devFdT* scalars, size_t 
 
 // include/operator/msm.cuh:26-46 (G2 records, two per point); uni_raw, the
 // reference's device copy of the generator, is accepted and unused
-template <typename dG2, typename devFdT>
+template <typename devFdT, typename dG2>
 void fix_base_multi_scalar_multiplication_g2(dG2* dest, devFdT* scalars, size_t length, dG2* uni_raw = nullptr) {
     (void)uni_raw;
     lightzk::run([&](hzk_ctx* c) {

@@ -1,19 +1,9 @@
-// Drop-in for the reference's include/operator/poly.cuh and the CSR
-// descriptor of include/data-structure/xpu-spmat.cuh:20-25.
+// Drop-in for the reference's include/operator/poly.cuh (CSR descriptors from
+// include/data-structure/xpu-spmat.hpp).
 #pragma once
 #include <cstddef>
+#include "../data-structure/xpu-spmat.hpp"
 #include "../lightzk_context.hpp"
-
-namespace xpu {
-// include/data-structure/xpu-spmat.cuh:20-25 (device arrays; row_ptr holds
-// `row` entries and no sentinel, the last row ends at cnt)
-template <typename FieldT>
-struct spmat_descript {
-    FieldT* val;
-    size_t *col_idx, *row_ptr;
-    size_t row, col, cnt;
-};
-}  // namespace xpu
 
 namespace lightzk {
 template <typename T>

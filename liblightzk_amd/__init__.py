@@ -468,6 +468,18 @@ def gen_field(seed, stream, n, kind="fr"):
     return out
 
 
+def ntt_setup(M):
+    """The reference's ntt_setup table (include/protocol/groth16-operator.cuh:6-32)
+    for an M-point domain: 6M + 2 Fr elements on the host (numpy uint64, 4 limbs each)."""
+    if M < 1 or M & (M - 1) or M > 1 << 26:
+        raise HzkError(f"ntt_setup: M={M} must be a power of two <= 2^26")
+    out = np.empty(4 * (6 * M + 2), dtype=np.uint64)
+    rc = load().hzk_ntt_setup(M, out.ctypes.data)
+    if rc != 0:
+        raise HzkError(f"hzk_ntt_setup(M={M}) rc={rc}: M must be a power of two <= 2^26")
+    return out
+
+
 def number_theory_transformation(ctx, deg, log_deg, coeffs, omega_powers, factor=None):
     """Reference operator poly.cuh:65-73: coeffs <- factor * DFT(coeffs) with w = omega_powers[1]."""
     if deg != 1 << log_deg:

@@ -52,6 +52,7 @@ def load():
         "hzk_msm_g2": (i32, [vp, vp, vp, sz, u32, sz, vp]),
         "hzk_ntt": (i32, [vp, vp, sz, vp, i32, vp]),
         "hzk_coset_ntt": (i32, [vp, vp, sz, i32]),
+        "hzk_ntt_setup": (i32, [sz, vp]),
         "hzk_ntt_batch": (i32, [vp, vp, sz, u32]),
         "hzk_msm_g1_batch": (i32, [vp, vp, sz, u32]),
         "hzk_msm_g2_batch": (i32, [vp, vp, sz, u32]),

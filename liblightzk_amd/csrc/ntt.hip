@@ -520,6 +520,41 @@ int hzk_ntt_batch(hzk_ctx* ctx, const hzk_ntt_desc* d, size_t count, uint32_t fl
     return ntt_batch(ctx, jobs.data(), count, (flags & HZK_BATCH_GRAPH) ? 1 : 0);
 }
 
+// The reference's twiddle table for an M-point interpolation domain and its
+// 2M-point evaluation domain (ntt_setup, include/protocol/groth16-operator.cuh:
+// 6-32), on the host: w_M^i and w_M^-i (i < M), M^-1, w_2M^i and w_2M^-i
+// (i < 2M), then (4M)^-1 at 6M + 1, the entry the reference writes there
+// (its layout comment, groth16.cuh:32-33, names (2M)^-1).
+int hzk_ntt_setup(size_t M, void* host_out) {
+    int lg = log2_exact(M);
+    if (lg < 0 || lg + 2 > 28 || !host_out) return HZK_ERR_INVALID;
+    host::Fr* t = (host::Fr*)host_out;
+    const host::Fr w[4] = {root_of_unity(lg), root_of_unity(lg).inverse(), root_of_unity(lg + 1),
+                           root_of_unity(lg + 1).inverse()};
+    const size_t base[4] = {0, M, 2 * M + 1, 4 * M + 1}, len[4] = {M, M, 2 * M, 2 * M};
+    const size_t chunk = 4096;
+    for (int s = 0; s < 4; ++s) {
+        const long nchunks = (long)((len[s] + chunk - 1) / chunk);
+#pragma omp parallel for schedule(static)
+        for (long c = 0; c < nchunks; ++c) {
+            size_t i0 = (size_t)c * chunk, i1 = std::min(len[s], i0 + chunk);
+            uint64_t e[4] = {(uint64_t)i0, 0, 0, 0};
+            host::Fr acc = w[s].pow(e);
+            for (size_t i = i0; i < i1; ++i) {
+                t[base[s] + i] = acc;
+                acc = acc * w[s];
+            }
+        }
+    }
+    host::Fr two_inv = fr_from_u64(2).inverse();
+    host::Fr mi = host::Fr::one(), m4i;
+    for (int i = 0; i < lg; ++i) mi = mi * two_inv;
+    m4i = mi * two_inv * two_inv;
+    t[2 * M] = mi;
+    t[6 * M + 1] = m4i;
+    return 0;
+}
+
 int hzk_coset_ntt(hzk_ctx* ctx, void* coeffs, size_t deg, int inverse) {
     if (!ctx) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, coeffs != nullptr, HZK_ERR_INVALID, "coset_ntt: null coefficients");

@@ -1,6 +1,7 @@
 // Exercises the reference-shaped C++ API (include/operator/*.hpp,
-// include/protocol/groth16.hpp) the way the reference's tests call the
-// operators (test/msm-check.cu, test/msm-g2-check.cu, test/groth16.cu), on
+// include/protocol/groth16.hpp) with explicit template arguments, the way the
+// reference's tests call the operators (test/msm-check.cu:28-39,
+// test/msm-g2-check.cu:38-49, test/groth16.cu:6-8), on
 // inputs written by tests/test_gpu_shims.py, and writes the outputs back for
 // that test to compare with the oracle and the C ABI.
 //
@@ -14,6 +15,7 @@
 #include <iterator>
 #include <string>
 #include <vector>
+#include "alt_bn128_types.hpp"
 #include "operator/msm.hpp"
 #include "operator/poly.hpp"
 #include "protocol/groth16.hpp"
@@ -46,9 +48,9 @@ struct dev {
     }
 };
 
-struct fr { uint8_t b[32]; };
-struct g1rec { uint8_t b[96]; };   // the reference's 96-byte device G1 record
-struct g2rec { uint8_t b[96]; };   // one lane record of a G2 point
+typedef alt_bn128::fr_t fr;
+typedef alt_bn128::g1_t g1rec;   // the reference's 96-byte device G1 record
+typedef alt_bn128::g2_t g2rec;   // one lane record of a G2 point
 
 }  // namespace
 
@@ -64,9 +66,9 @@ int main(int argc, char** argv) {
         // MSM (test/msm-check.cu:28-39 order: fixed-base table, pre_comp levels, MSM)
         dev ks(slurp("ks.bin")), sc(slurp("scalars.bin"));
         dev t1(96 * n * level), d1(96);
-        fix_base_multi_scalar_multiplication_g1((g1rec*)t1.p, (fr*)ks.p, n);
-        pre_comp_g1(n, (g1rec*)t1.p, gap, level);
-        multi_scalar_multiplication_g1(n, (fr*)sc.p, (g1rec*)t1.p, (g1rec*)d1.p, 12, 22, 4096, n);
+        (fix_base_multi_scalar_multiplication_g1<fr, g1rec>)((g1rec*)t1.p, (fr*)ks.p, n);
+        (pre_comp_g1<g1rec>)(n, (g1rec*)t1.p, gap, level);
+        multi_scalar_multiplication_g1<fr, g1rec>(n, (fr*)sc.p, (g1rec*)t1.p, (g1rec*)d1.p, 12, 22, 4096, n);
         spill("msm_g1.bin", d1.get(96));
         {
             std::vector<uint8_t> lv;
@@ -77,9 +79,9 @@ int main(int argc, char** argv) {
             spill("levels_g1.bin", lv);
         }
         dev t2(192 * n * level), d2(192);
-        fix_base_multi_scalar_multiplication_g2((g2rec*)t2.p, (fr*)ks.p, n);
-        pre_comp_g2(n, (g2rec*)t2.p, gap, level);
-        multi_scalar_multiplication_g2(n, (fr*)sc.p, (g2rec*)t2.p, (g2rec*)d2.p, 12, 22, 4096, n);
+        (fix_base_multi_scalar_multiplication_g2<fr, g2rec>)((g2rec*)t2.p, (fr*)ks.p, n);
+        (pre_comp_g2<g2rec>)(n, (g2rec*)t2.p, gap, level);
+        multi_scalar_multiplication_g2<fr, g2rec>(n, (fr*)sc.p, (g2rec*)t2.p, (g2rec*)d2.p, 12, 22, 4096, n);
         spill("msm_g2.bin", d2.get(192));
         {
             std::vector<uint8_t> lv;
@@ -93,7 +95,7 @@ int main(int argc, char** argv) {
         dev x(slurp("ntt_in.bin")), tab(slurp("ntt_tab.bin")), fac(slurp("ntt_factor.bin"));
         int log_deg = 0;
         while (((size_t)1 << log_deg) < deg) ++log_deg;
-        number_theory_transformation(deg, (size_t)log_deg, (fr*)x.p, (fr*)tab.p, (fr*)fac.p);
+        number_theory_transformation<fr>(deg, (size_t)log_deg, (fr*)x.p, (fr*)tab.p, (fr*)fac.p);
         spill("ntt_out.bin", x.get(32 * deg));
         // SpMV and A o B - C (groth16.cuh:160-162, :188)
         const char* mats[3] = {"a", "b", "c"};
@@ -108,23 +110,23 @@ int main(int argc, char** argv) {
             d[m] = {(fr*)v->p, (size_t*)c->p, (size_t*)r->p, M, nv, nnz[m]};
         }
         dev z(slurp("z.bin")), az(32 * M), bz(32 * M), cz(32 * M);
-        batch_sparse_matrix_vector_multiplication(d[0], d[1], d[2], (fr*)z.p, (fr*)az.p, (fr*)bz.p, (fr*)cz.p);
+        (batch_sparse_matrix_vector_multiplication<fr>)(d[0], d[1], d[2], (fr*)z.p, (fr*)az.p, (fr*)bz.p, (fr*)cz.p);
         std::vector<uint8_t> out = az.get(32 * M), tb = bz.get(32 * M), tc = cz.get(32 * M);
         out.insert(out.end(), tb.begin(), tb.end());
         out.insert(out.end(), tc.begin(), tc.end());
         spill("spmv.bin", out);
-        polynomial_A_times_B_minus_C(M, (fr*)az.p, (fr*)bz.p, (fr*)cz.p);
+        (polynomial_A_times_B_minus_C<fr>)(M, (fr*)az.p, (fr*)bz.p, (fr*)cz.p);
         spill("abc.bin", az.get(32 * M));
         for (dev* p : keep) delete p;
         // protocol class (test/groth16.cu:3-8)
-        Groth16 g(input_scale::MatMul_p_39);
+        Groth16_BN254 g(input_scale::MatMul_p_39);
         g.Setup(12);
         g.Prove(1);
         spill("g16_proof.bin", std::vector<uint8_t>(g.proof().begin(), g.proof().end()));
         // error behaviour: an exception carrying the library's message
         std::string err = "none";
         try {
-            number_theory_transformation(12, 3, (fr*)x.p, (fr*)tab.p, (fr*)nullptr);
+            number_theory_transformation<fr>(12, 3, (fr*)x.p, (fr*)tab.p, (fr*)nullptr);
         } catch (const lightzk::error& e) {
             err = e.what();
         }

@@ -63,3 +63,16 @@ def test_builtin_plan_table_matches_json(tmp_path):
     # two regimes x two groups x 32 size classes
     rows = [ln for ln in shipped.splitlines() if ln.startswith("{{")]
     assert len(rows) == 4 and all(r.count("u}") == 32 for r in rows)
+
+
+def test_ntt_setup_table_matches_oracle(orc):
+    """hzk_ntt_setup (host-side, no GPU) writes the reference's ntt_setup table
+    (include/protocol/groth16-operator.cuh:6-32) entry for entry, including
+    (4M)^-1 at 6M + 1; the oracle's table is pinned to the reference's root
+    tables (tests/test_oracle_ntt_spmv.py)."""
+    import numpy as np
+    for M in (1, 2, 1 << 5, 1 << 12):
+        assert np.array_equal(liblightzk_amd.ntt_setup(M), orc.ntt_setup(M)), M
+    for bad in (0, 3, 1 << 27):
+        with pytest.raises(liblightzk_amd.HzkError):
+            liblightzk_amd.ntt_setup(bad)
