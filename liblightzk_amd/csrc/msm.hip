@@ -512,6 +512,7 @@ MsmPlan msm_plan(size_t n, int force_c, int force_g, size_t lds_bytes) {
     if (g > 16) g = 16;
     if (g > p.NB) g = p.NB;
     if (force_g) g = force_g;
+    if (g > p.NB) g = p.NB;   // a run never spans more buckets than a window has
     p.g = g;
     // scalar tiles per window for the LDS histogram / scatter workgroups
     // (about 512 workgroups per MSM; every tile at least 4096 scalars and 4x

@@ -285,21 +285,18 @@ struct Oracle16 {
     Circuit c;
     KeyScalars ks;
     bool have_points = false;
+    bool have_vk = false;
     std::vector<G1> A1, B1, L1, H1, IC1;
     std::vector<G2> B2;
     G1 alpha1, beta1, delta1;
     G2 beta2, gamma2, delta2;
 };
 
-void ensure_points(Oracle16& o) {
-    if (o.have_points) return;
+// Verifying-key points only (IC and the six toxic-waste points): what the
+// pairing check needs, without the O(m) proving-key arrays.
+void ensure_vk(Oracle16& o) {
+    if (o.have_vk) return;
     static FixedBase<G1> fb1(g1_generator());
-    static FixedBase<G2> fb2(g2_generator());
-    points_from_scalars<G1, Fq>(o.ks.At, fb1, o.A1);
-    points_from_scalars<G1, Fq>(o.ks.Bt, fb1, o.B1);
-    points_from_scalars<G2, Fq2>(o.ks.Bt, fb2, o.B2);
-    points_from_scalars<G1, Fq>(o.ks.K, fb1, o.L1);
-    points_from_scalars<G1, Fq>(o.ks.H, fb1, o.H1);
     points_from_scalars<G1, Fq>(o.ks.IC, fb1, o.IC1);
     o.alpha1 = mul1(o.ks.t.alpha, g1_generator()).to_affine();
     o.beta1 = mul1(o.ks.t.beta, g1_generator()).to_affine();
@@ -307,6 +304,19 @@ void ensure_points(Oracle16& o) {
     o.beta2 = mul2(o.ks.t.beta, g2_generator()).to_affine();
     o.gamma2 = mul2(o.ks.t.gamma, g2_generator()).to_affine();
     o.delta2 = mul2(o.ks.t.delta, g2_generator()).to_affine();
+    o.have_vk = true;
+}
+
+void ensure_points(Oracle16& o) {
+    if (o.have_points) return;
+    ensure_vk(o);
+    static FixedBase<G1> fb1(g1_generator());
+    static FixedBase<G2> fb2(g2_generator());
+    points_from_scalars<G1, Fq>(o.ks.At, fb1, o.A1);
+    points_from_scalars<G1, Fq>(o.ks.Bt, fb1, o.B1);
+    points_from_scalars<G2, Fq2>(o.ks.Bt, fb2, o.B2);
+    points_from_scalars<G1, Fq>(o.ks.K, fb1, o.L1);
+    points_from_scalars<G1, Fq>(o.ks.H, fb1, o.H1);
     o.have_points = true;
 }
 
@@ -518,7 +528,7 @@ int orc_g16_prove(void* h, const uint64_t* z, const uint64_t* rs, uint64_t* proo
 // Pairing verifier; returns 1 if the proof is accepted.
 int orc_g16_verify(void* h, const uint64_t* public_z /* num_public+1 values, z[0]=1 */, const uint64_t* proof) {
     Oracle16* o = static_cast<Oracle16*>(h);
-    ensure_points(*o);
+    ensure_vk(*o);
     G1 vkx = G1::zero();
     for (size_t i = 0; i <= o->c.num_public; ++i) vkx = vkx + mul1(Fr::from_raw(public_z + 4 * i), o->IC1[i]);
     uint64_t ps[4 * 8], qs[4 * 16];

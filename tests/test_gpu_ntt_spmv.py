@@ -116,6 +116,25 @@ def test_spmv3_and_pointwise(ctx, orc):
     assert not outs[0].download(np.uint64, 4 * M).any()
 
 
+@pytest.mark.parametrize("deg", [1, 7, 1000, 1 << 16])
+def test_pointwise_abc_random(ctx, orc, deg):
+    """A <- A o B - C (poly.cuh:75-86) on random, unrelated A, B, C: the
+    oracle's Fr product and difference, element by element; B and C unchanged.
+    Boundary values (0, 1, r - 1) lead the vectors."""
+    import liblightzk_amd as z
+    a, b, c = (orc.gen_fr(90 + i, 0, deg) for i in range(3))
+    edge = [pm.int_to_limbs(v) for v in (0, pm.MONT % pm.R, pm.R - 1)]
+    for j, e in enumerate(edge[: min(3, deg)]):
+        a[4 * j: 4 * j + 4] = e
+        b[4 * (deg - 1 - j): 4 * (deg - j)] = e
+    da, db, dc = ctx.upload(a), ctx.upload(b), ctx.upload(c)
+    z.polynomial_A_times_B_minus_C(ctx, deg, da, db, dc)
+    want = orc.field_op("fr", 2, orc.field_op("fr", 0, a, b), c)
+    assert np.array_equal(da.download(np.uint64, 4 * deg), want)
+    assert np.array_equal(db.download(np.uint64, 4 * deg), b)
+    assert np.array_equal(dc.download(np.uint64, 4 * deg), c)
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_ntt_batch_uneven_sizes_and_kinds(ctx, orc, graph):
     """Batched transforms (one launch per pass round over all transforms,

@@ -194,3 +194,19 @@ def test_batch_autotune_small_classes(orc, data, tmp_path):
         assert tune.load(c3, path) == len(res)
         for lg, v in res.items():
             assert c3.msm_batch_tune_get(1, lg) == tuple(v)
+
+
+@pytest.mark.parametrize("c,segment", [(7, 128), (5, 4096), (4, 16)])
+def test_forced_segment_longer_than_window(ctx, orc, data, c, segment):
+    """hzk_set_tuning with a run length above a window's 2^(c-1) buckets is
+    clamped to the window: the MSM stays exact (it returned infinity before)."""
+    n, s, _, k, b1, b2 = data
+    ctx.set_tuning(c, segment)
+    try:
+        ds, db1, db2, d1, d2 = ctx.upload(s), ctx.upload(b1), ctx.upload(b2), ctx.alloc(96), ctx.alloc(192)
+        ctx.msm_g1(ds, db1, n, d1)
+        ctx.msm_g2(ds, db2, n, d2)
+        assert np.array_equal(d1.download(np.uint64, 12), orc.msm_closed_form(s, k, 1))
+        assert np.array_equal(d2.download(np.uint64, 24), orc.msm_closed_form(s, k, 2))
+    finally:
+        ctx.set_tuning(0, 0)
