@@ -4,15 +4,8 @@
 #include <vector>
 #include "common.hpp"
 #include "host_ec.hpp"
+#include "keyfile_format.hpp"
 #include "msm.hpp"
-
-namespace hzk {
-struct HostCsr {
-    std::vector<uint64_t> row_ptr, col;
-    std::vector<host::Fr> val;
-};
-
-}  // namespace hzk
 
 struct hzk_g16 {
     int log_m = 0;
@@ -51,8 +44,17 @@ struct hzk_g16 {
     void *A1L = nullptr, *C1L = nullptr, *B2L = nullptr;
     int lc = 0;
     uint32_t lv = 1;
-    std::vector<void*> owned;
+    std::vector<void*> owned;        // device allocations freed with the key
+    std::vector<size_t> owned_bytes; // their sizes (resident key bytes = the sum)
     int device = 0;
+    void own(void* p, size_t bytes) { owned.push_back(p), owned_bytes.push_back(bytes); }
+    void disown(void* p) {
+        for (size_t i = 0; i < owned.size(); ++i)
+            if (owned[i] == p) {
+                owned.erase(owned.begin() + (long)i), owned_bytes.erase(owned_bytes.begin() + (long)i);
+                return;
+            }
+    }
 };
 
 namespace hzk {

@@ -151,7 +151,7 @@ template <class T>
 int dev_upload(hzk_ctx* ctx, hzk_g16* g, const T* src, size_t bytes, void** out) {
     void* d = nullptr;
     HZK_HIP(ctx, hipMalloc(&d, bytes ? bytes : 1));
-    g->owned.push_back(d);
+    g->own(d, bytes);
     if (bytes) HZK_HIP(ctx, hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
     *out = d;
     return 0;
@@ -160,7 +160,7 @@ int dev_upload(hzk_ctx* ctx, hzk_g16* g, const T* src, size_t bytes, void** out)
 int dev_alloc(hzk_ctx* ctx, hzk_g16* g, size_t bytes, void** out) {
     void* d = nullptr;
     HZK_HIP(ctx, hipMalloc(&d, bytes ? bytes : 1));
-    g->owned.push_back(d);
+    g->own(d, bytes);
     *out = d;
     return 0;
 }
@@ -203,7 +203,7 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
         if (*p) {
             (void)hipStreamSynchronize(ctx->stream);
             (void)hipFree(*p);
-            g->owned.erase(std::remove(g->owned.begin(), g->owned.end(), *p), g->owned.end());
+            g->disown(*p);
             *p = nullptr;
         }
     }
@@ -549,8 +549,8 @@ void hzk_g16_free(hzk_g16* g) {
 
 int hzk_g16_info(const hzk_g16* g, uint64_t* out8) {
     if (!g || !out8) return HZK_ERR_INVALID;
-    uint64_t kb = g->num_vars * (64 + 64 + 128) + (g->num_vars - g->num_public - 1) * 64 + (g->M - 1) * 64;
-    kb *= g->lv > 1 ? 1 + g->lv : 1;   // plus the level tables
+    uint64_t kb = 0;   // every device array of the key: CSR, full and compacted query arrays, level tables, buffers
+    for (size_t b : g->owned_bytes) kb += b;
     uint64_t v[8] = {g->M, g->num_vars, g->num_public, g->num_gates, g->m[0].col.size(), g->m[1].col.size(),
                      g->m[2].col.size(), kb};
     std::memcpy(out8, v, sizeof v);

@@ -13,9 +13,11 @@
 //     concurrently on two streams:
 //       stage 1  newly admitted requests: upload z, SpMV, and the H pipeline
 //                with every request's transforms in shared batched launches;
-//       stage 2  requests whose H finished in the previous step: their ten
-//                MSMs as one batched G1 launch sequence (4 per request) and one
-//                G2 sequence, then the host assembly.
+//       stage 2  requests whose H finished in the previous step: their three
+//                MSMs (G1: the compacted A query and the merged L | H | B1 MSM
+//                of C; G2: the compacted B query) as one batched G1 launch
+//                sequence (2 per request) and one G2 sequence, then the host
+//                assembly.
 //     Requests that arrive while others are on the GPU join at the next step,
 //     so the device always holds the H work of the newest requests next to the
 //     MSM work of the previous ones.
@@ -77,6 +79,7 @@ struct hzk_server {
     int mode = 0;
     int max_batch = 64;
     size_t budget = 0, in_use = 0;
+    bool auto_budget = false;   // budget from free memory at start (after the keys are resident)
     std::vector<std::unique_ptr<Pipeline>> pipes;
     std::vector<hzk_g16*> circuits;
     // finished requests' buffers, kept per circuit for reuse (hipFree would
@@ -379,9 +382,8 @@ int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hz
         delete s;
         return rc;
     }
-    size_t fr = 0, tot = 0;
-    (void)hipMemGetInfo(&fr, &tot);
-    s->budget = mem_budget ? mem_budget : fr / 2;
+    s->budget = mem_budget;
+    s->auto_budget = mem_budget == 0;
     *out = s;
     return HZK_OK;
 }
@@ -470,6 +472,12 @@ int hzk_server_set_mode(hzk_server* s, int mode) {
 
 int hzk_server_start(hzk_server* s) {
     if (!s || s->running) return HZK_ERR_INVALID;
+    if (s->auto_budget) {
+        size_t fr = 0, tot = 0;
+        (void)hipSetDevice(s->device);
+        (void)hipMemGetInfo(&fr, &tot);
+        s->budget = fr / 2;
+    }
     s->running = true;
     for (auto& p : s->pipes) p->th = std::thread(worker_loop, s, p.get());
     return HZK_OK;
