@@ -8,14 +8,21 @@ Pippenger MSM (hzk_msm_g1: recode, LDS counting sort, bucket accumulation,
 merge, bucket reduction, window combine, normalisation) over all 2^24 points.
 
 Prints ONE JSON line. `value` is Mpoints/s (n / seconds per step).
-`roofline` is for the dominant kernel (msm_accumulate): algorithmic 32x32-bit
-integer multiplies per launch = entries * 11 * 128 (one mixed addition of
-11 Fq products per (point, window) entry, 128 32x32 products per Fq product),
-divided by the kernel's average launch time from HIP events on the
-context's stream, against the measured v_mad_u64_u32 issue rate.
+`roofline` is for the dominant kernel (msm_accumulate): the algorithmic
+32x32-bit integer multiplies of the MSM, W_MSM(n) = min_c ceil(255/c)
+(11 n + 16 2^c) * 128 (signed-digit Pippenger: 11 Fq products per point and
+window, 16 per bucket of the running-sum reduction, 128 32x32 products per
+Fq product; independent of the plan this build runs), divided by the
+kernel's average launch time from HIP events on the context's stream,
+against the full-rate VALU integer-multiply peak of the MI355X (256 CUs x 64
+lanes x 2.4 GHz = 3.93e13/s); the measured v_mad_u64_u32 issue rate is
+reported beside it. The NTT and plain-proof legs carry roofline blocks of
+the same form (W_NTT(n) = (n/2) log2 n * 128; a proof's W = its five MSMs'
+W_MSM, G2 at 3 Fq products per Fq2 product, plus seven W_NTT(M)).
 `cpu_baseline` times the oracle's naive MSM (per-thread s_i * P_i
 accumulators, test/msm-check.cu:46-55 semantics) on a bounded sample of the
-same inputs on this host's cores.
+same inputs on this host's cores, and the oracle's Groth16 prover on the
+m = 2^18 circuit of the plain-proof leg.
 """
 import argparse
 import json
@@ -33,15 +40,48 @@ METRIC = "G1 MSM Mpoints/s @2^24; Groth16 proofs/s + p50/p99 latency, served vs 
 INTMUL_PER_FQ = 128          # 8x8 product + 8x8 reduction, 32x32->64 each
 FQ_PER_MADD = 11             # madd-2007-bl, the algorithmic count used for MSM work
 HBM_PEAK = 8.0e12            # MI355X published HBM3E bandwidth, bytes/s
+INTMUL_PEAK = 256 * 64 * 2.4e9   # MI355X: 256 CUs x 64 VALU lanes x 2.4 GHz, one 32x32 multiply per lane per clock
 
 
-def w_msm(n):
-    """Algorithmic int-mul count of a G1 MSM: min_c ceil(255/c) (11 n + 16 2^c) M_Fq."""
+def w_msm(n, group=1):
+    """Algorithmic int-mul count of an MSM: min_c ceil(255/c) (11 n + 16 2^c) M, M = M_Fq (G1) or 3 M_Fq (G2)."""
     best = None
     for c in range(1, 25):
         w = math.ceil(255 / c) * (11 * n + 16 * 2 ** c) * INTMUL_PER_FQ
         best = w if best is None or w < best else best
-    return best
+    return best * (3 if group == 2 else 1)
+
+
+def w_ntt(n):
+    """Algorithmic int-mul count of one n-point NTT: (n/2) log2 n M_Fr."""
+    return (n // 2) * int(math.log2(n)) * INTMUL_PER_FQ
+
+
+def w_proof(num_vars, num_public, M):
+    """A Groth16 proof's algorithmic int-mul count: G1 MSMs over A, B1 (num_vars
+    each), L (num_vars - num_public - 1) and H (M - 1), the G2 MSM over B
+    (num_vars), and seven M-point NTTs (3 inverse, 3 coset, 1 coset inverse)."""
+    return (2 * w_msm(num_vars) + w_msm(num_vars - num_public - 1) + w_msm(M - 1) + w_msm(num_vars, 2) + 7 * w_ntt(M))
+
+
+def intmul_roofline(work, seconds, kernel, extra=None):
+    achieved = work / seconds if seconds > 0 else 0.0
+    r = {"bound": "valu-intmul", "kernel": kernel, "achieved": achieved / 1e12, "peak": INTMUL_PEAK / 1e12,
+         "unit": "T int-mul/s (32x32->64)", "frac": achieved / INTMUL_PEAK, "work_per_launch": work,
+         "peak_source": "256 CUs x 64 lanes x 2.4 GHz (MI355X published CU count and peak clock)"}
+    if extra:
+        r.update(extra)
+    return r
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
@@ -58,12 +98,13 @@ def cpu_baseline(ctx, scalars_host, bases_dev, budget_s):
     t = time.perf_counter()
     orc.msm_g1(s, b, 0)
     el = time.perf_counter() - t
-    return {"value": n_s / el / 1e6, "unit": "Mpoints/s", "cores": nthreads, "kind": "port",
+    return {"value": n_s / el / 1e6, "unit": "Mpoints/s", "cores": nthreads, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"first {n_s} points of the same workload, naive per-thread double-and-add MSM "
-                      f"(test/msm-check.cu:46-55 semantics), {el:.1f} s"}
+                      f"(test/msm-check.cu:46-55 semantics), {el:.1f} s; the cost is linear in n, so the "
+                      f"2^24 rate equals this one"}
 
 
-def cpu_prover_baseline(ctx, log_m=16, proofs=2):
+def cpu_prover_baseline(ctx, log_m=18, proofs=2):
     """Part of the cpu_baseline leg: the oracle's honest Groth16 prover (CPU
     SpMV, radix-2 NTTs, CPU Pippenger MSMs, phase-6 assembly; the phase order
     of include/protocol/groth16.cuh:154-222) on a bounded circuit, next to the
@@ -83,7 +124,9 @@ def cpu_prover_baseline(ctx, log_m=16, proofs=2):
     gpu_s = (time.perf_counter() - t) / proofs
     g.close()
     return {"value": 1.0 / cpu_s, "unit": "proofs/s", "cores": orc.lib().orc_num_threads(), "kind": "port",
-            "sample": f"{proofs} proofs of the m=2^{log_m} multiplication-gate circuit (CPU Pippenger MSMs)",
+            "cpu_model": cpu_model(),
+            "sample": f"{proofs} proofs of the m=2^{log_m} multiplication-gate circuit (seed 2, the plain-proof "
+                      f"leg's circuit; CPU SpMV, radix-2 NTTs, CPU Pippenger MSMs; key points built before the clock)",
             "gpu_plain_same_circuit_proofs_per_s": 1.0 / gpu_s,
             "proofs_identical": bool(all(np.array_equal(a, b) for a, b in zip(cpu, gpu)))}
 
@@ -109,9 +152,12 @@ def groth16_plain(ctx, log_m, proofs, levels=1):
     el = time.perf_counter() - t0
     rep = ctx.timing_report()
     ctx.timing(False)
+    info = g.info
     g.close()
     lat.sort()
+    work = w_proof(info["num_vars"], info["num_public"], info["M"])
     return {"config": f"m=2^{log_m} multiplication-gate circuit, {g.info['num_vars']} variables",
+            "roofline": intmul_roofline(work, el / proofs, "whole proof (wall clock per proof)"),
             "proofs": proofs, "proofs_per_s": proofs / el, "p50_ms": lat[len(lat) // 2] * 1e3,
             "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))] * 1e3, "setup_s": setup_s, "key_levels": levels,
             "kernels_ms_per_proof": {k: v[0] / proofs for k, v in rep.items()}}
@@ -248,6 +294,17 @@ def batched_forms(ctx, count=128, seed=4, reps=3):
     n1 = (time.perf_counter() - t) / reps
     ntt_same = bool(np.array_equal(d_a.download(np.uint64, 4 * tot), d_c.download(np.uint64, 4 * tot)))
     bfly = sum((1 << lg) // 2 * lg for lg in logs)
+    # one 2^18 transform at a time (the size of a proof's transforms)
+    n18 = 1 << 18
+    d_18 = ctx.upload(x[: 4 * n18])
+    ctx.ntt(d_18, n18)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(20):
+        ctx.ntt(d_18, n18)
+    ctx.sync()
+    t18 = (time.perf_counter() - t) / 20
+    d_18.free()
     for b in (d_s, d_k, d_b, outs_b, outs_1, d_a, d_c):
         b.free()
     return {"msm": {"instances": count, "points": pts, "sizes": "floor(2^(10+10U)), seeded",
@@ -257,7 +314,13 @@ def batched_forms(ctx, count=128, seed=4, reps=3):
             "ntt": {"transforms": count, "elements": tot, "sizes": "2^k, k uniform in 10..20, seeded",
                     "batched_graph_ms": nb * 1e3, "one_at_a_time_ms": n1 * 1e3,
                     "batched_Gbutterflies_per_s": bfly / nb / 1e9, "one_at_a_time_Gbutterflies_per_s": bfly / n1 / 1e9,
-                    "speedup": n1 / nb, "outputs_identical": ntt_same}}
+                    "speedup": n1 / nb, "outputs_identical": ntt_same,
+                    "single_2^18_us": t18 * 1e6, "single_2^18_Gbutterflies_per_s": (n18 // 2) * 18 / t18 / 1e9,
+                    "roofline": intmul_roofline(bfly * INTMUL_PER_FQ, nb, "batched NTT (whole batch, wall clock)",
+                                                {"hbm_achieved_GBps": tot * 64 / nb / 1e9, "hbm_peak_GBps": HBM_PEAK / 1e9,
+                                                 "hbm_frac": tot * 64 / nb / HBM_PEAK,
+                                                 "algorithmic_bytes": tot * 64,
+                                                 "single_2^18_frac": w_ntt(n18) / t18 / INTMUL_PEAK})}}
 
 
 def large_msm_variants(ctx, reps=2):
@@ -478,31 +541,30 @@ def main():
 
     acc_ms, acc_n = rep.get("msm_accumulate", (0.0, 0))
     acc_avg_s = acc_ms / max(acc_n, 1) * 1e-3
-    work = st["entries"] * FQ_PER_MADD * INTMUL_PER_FQ
-    achieved = work / acc_avg_s if acc_avg_s > 0 else 0.0
+    work = w_msm(n)
+    entry_work = st["entries"] * FQ_PER_MADD * INTMUL_PER_FQ
     traffic = None   # HBM bytes per accumulate launch, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
     prof = os.path.join(ROOT, "profiles", "msm_g1_2e24_hbm_traffic.txt")
     if args.log_n == 24 and os.path.exists(prof):
         for line in open(prof):
-            if line.startswith("k_msm_accumulate<Fq >"):
+            if line.startswith("k_msm_accumulate"):
                 traffic = float(line.split()[-1]) * 1e9
-    roofline = {
-        "bound": "valu-intmul",
-        "kernel": "msm_accumulate",
-        "achieved": achieved / 1e9,
-        "peak": peak_mad / 1e9,
-        "unit": "G int-mul/s",
-        "frac": achieved / peak_mad if peak_mad else None,
+    roofline = intmul_roofline(work, acc_avg_s, "msm_accumulate", {
         "traffic": traffic,
         "traffic_source": "profiles/msm_g1_2e24_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)",
-        "peak_source": "measured v_mad_u64_u32 issue rate on this GPU (hzk_roofline_probe)",
-        "work_per_launch": work,
+        "work_formula": "W_MSM(n) = min_c ceil(255/c) (11 n + 16 2^c) * 128",
         "avg_launch_ms": acc_avg_s * 1e3,
-        "msm_intmul_frac": (w_msm(n) / (el / args.steps)) / peak_mad if peak_mad else None,
+        "measured_peak": peak_mad / 1e12,
+        "frac_of_measured_peak": (work / acc_avg_s) / peak_mad if peak_mad and acc_avg_s > 0 else None,
+        "measured_peak_source": "v_mad_u64_u32 issue rate measured on this GPU (hzk_roofline_probe)",
+        "entry_work_per_launch": entry_work,
+        "entry_frac": (entry_work / acc_avg_s) / INTMUL_PEAK if acc_avg_s > 0 else None,
+        "entry_formula": "this plan's nonzero bucket entries x 11 x 128",
+        "whole_msm_frac": (work / (el / args.steps)) / INTMUL_PEAK,
         "algorithmic_bytes_per_step": n * 96,
         "hbm_frac": n * 96 / (el / args.steps) / HBM_PEAK,
         "hbm_copy_measured_GBps": hbm_copy / 1e9,
-    }
+    })
     out = {
         "metric": METRIC,
         "value": value,

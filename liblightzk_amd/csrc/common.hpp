@@ -121,7 +121,8 @@ int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 struct NttJob {
     void* x;
     size_t deg;
-    int kind;   // 0 NTT, 1 inverse NTT, 2 coset NTT, 3 coset inverse NTT
+    int kind;   // 0 NTT, 1 inverse NTT, 2 coset NTT, 3 coset inverse NTT (natural order);
+                // 4..6 the H pipeline's permutation-free forms (ntt.hip, ntt_batch)
 };
 int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture);
 // reference base tables (layout.hip): level 0 as affine in ctx->conv

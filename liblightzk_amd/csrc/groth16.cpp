@@ -298,10 +298,21 @@ int setup(hzk_ctx* ctx, hzk_g16* g, uint64_t seed) {
 
 }  // namespace
 
+// dst[p] = src[rev_k(p)], p < n, for 64-byte affine G1 points (n <= 2^k - 1,
+// so rev_k(p) < 2^k - 1 as well)
+__global__ void __launch_bounds__(256) k_gather_bitrev64(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n, int k) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // one 16-byte word
+    if (q >= 4 * n) return;
+    const size_t p = q >> 2;
+    const size_t r = __brev((uint32_t)p) >> (32 - k);
+    dst[q] = src[4 * r + (q & 3)];
+}
+
 // Compacted A and B query arrays: the variables whose A1 (B1) point is not
 // the point at infinity (B_i(tau) = 0 makes B1_i and B2_i infinity alike),
 // found on the host from the key arrays, then gathered on the device; and
-// the merged base array C1 = L1 | H1 | B1c of the C MSM.
+// the merged base array C1 = L1 | H1 (bit-reversed order, matching the
+// quotient h that the H pipeline leaves bit-reversed) | B1c of the C MSM.
 int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
     const size_t N1 = g->num_vars;
     std::vector<uint64_t> pts(N1 * 8);
@@ -329,11 +340,13 @@ int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
     if (!rc) rc = dev_alloc(ctx, g, g->nC * 64, &g->C1);
     if (!rc) rc = gather_rows(ctx, g->A1, g->idxA, g->nA, 64, g->A1c);
     if (!rc) rc = gather_rows(ctx, g->B2, g->idxB, g->nB, 128, g->B2c);
-    // C1 = L1 | H1 | B1 at idxB
+    // C1 = L1 | H1 bit-reversed | B1 at idxB
     if (!rc) rc = gather_rows(ctx, g->B1, g->idxB, g->nB, 64, (char*)g->C1 + (nL + nH) * 64);
     if (rc) return rc;
     HZK_HIP(ctx, hipMemcpyAsync(g->C1, g->L1, nL * 64, hipMemcpyDeviceToDevice, ctx->stream));
-    HZK_HIP(ctx, hipMemcpyAsync((char*)g->C1 + nL * 64, g->H1, nH * 64, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_gather_bitrev64, dim3(blocks_for(4 * nH, 256)), dim3(256), 0, ctx->stream, (const uint4*)g->H1,
+                       (uint4*)((char*)g->C1 + nL * 64), nH, g->log_m);
+    HZK_HIP(ctx, hipGetLastError());
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HZK_OK;
 }
@@ -362,7 +375,11 @@ int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n) {
         int rc = spmv3(ctx, &cs[0], &cs[1], &cs[2], reqs[i].z, reqs[i].a, reqs[i].b, reqs[i].c);
         if (rc) return rc;
     }
-    for (int kind : {1, 2}) {
+    // evaluations on <w> (natural order) -> coefficients (bit-reversed, times M)
+    // -> evaluations on g<w> (natural order; g^rev(p) / M applied on the way
+    // in) -> (A B - C) / (g^M - 1) -> coefficients of h, bit-reversed (g^-rev(p)
+    // / M applied on the way out): no bit-reverse permutation anywhere
+    for (int kind : {4, 5}) {
         jobs.clear();
         for (size_t i = 0; i < n; ++i)
             for (void* v : {reqs[i].a, reqs[i].b, reqs[i].c}) jobs.push_back({v, reqs[i].g->M, kind});
@@ -373,7 +390,7 @@ int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n) {
     for (size_t i = 0; i < n; ++i) {
         int rc = pointwise_abc(ctx, reqs[i].g->M, reqs[i].a, reqs[i].b, reqs[i].c, reqs[i].g->zinv);
         if (rc) return rc;
-        jobs.push_back({reqs[i].a, reqs[i].g->M, 3});
+        jobs.push_back({reqs[i].a, reqs[i].g->M, 6});
     }
     return ntt_batch(ctx, jobs.data(), jobs.size(), 0);
 }

@@ -9,12 +9,28 @@
 // include/protocol/groth16-operator.cuh:6-32) and the same decimation-in-
 // frequency butterfly (a, b) -> (a + b, (a - b) w^((j mod h) 2^s)).
 //
-// Here the log2(deg) stages are grouped into passes: each workgroup stages a
-// 2^11-element tile (64 KiB) in LDS — 2^R butterfly rows by G = 2^(11-R)
-// columns that are contiguous in HBM, so loads and stores stay coalesced —
-// runs R stages there, and writes the tile back. 2^20 takes two passes
-// (R = 9, then the last 11 stages, whose rows are contiguous). One more pass
-// applies the bit-reverse permutation fused with the scale factor.
+// Here the log2(deg) stages are grouped into passes. A pass covers R
+// consecutive stages over tiles of 2^R rows x 2^gbits columns (2^11 elements,
+// 64 KiB); columns are contiguous in HBM, so loads and stores stay coalesced.
+// Inside a tile every thread holds 8 elements in registers: the 2^3 rows
+// that three consecutive stages pair up (a radix-8 group, or a radix-4 /
+// radix-2 group padded with a neighbouring bit), runs those stages' 12
+// butterflies there with interleaved products, and trades its elements with
+// the other threads through LDS before the next three stages: 4 LDS
+// exchanges for 11 stages instead of 11 LDS round trips and barriers. 2^20
+// takes two passes (9 stages, then the last 11, whose rows are contiguous).
+//
+// Decimation in frequency (DIF, the reference's butterfly) takes natural
+// order to bit-reversed order; running the same stages backwards with the
+// transposed butterfly (a, b) -> (a + t b, a - t b) (decimation in time,
+// DIT) takes bit-reversed order back to natural order. The operator keeps
+// the reference's natural-in, natural-out contract with one more pass that
+// applies the bit-reverse permutation fused with the scale factor. The
+// Groth16 H pipeline needs no permutation at all: DIF for its inverse
+// transforms, DIT for its coset transform, with the coset powers and 1/n
+// folded into the first load or the last store of a pass, and the quotient
+// h left in bit-reversed order, matched by a bit-reversed copy of the H
+// query points.
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -26,66 +42,204 @@
 namespace hzk {
 
 static constexpr int NTT_TILE_BITS = 11;
-// 1024 threads (16 waves) per 64 KiB tile: the LDS allows two tiles per CU,
-// so the workgroup size sets the waves available to hide LDS and twiddle-load
-// latency (256 threads: 41.6 G butterflies/s batched, 12.5 one transform at a
-// time; 1024: 48.5 and 18.6)
-static constexpr int NTT_THREADS = 1024;
+static constexpr int NTT_EPT = 8;                                        // elements per thread
+static constexpr int NTT_THREADS = (1 << NTT_TILE_BITS) / NTT_EPT;       // 256
 static constexpr int NTT_MAX_LOG = 26;
 
-// one tile of a pass of stages [s0, s0 + R) over tiles of 2^R rows x 2^gbits columns
-HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k, int s0, int R, int gbits, uint32_t b, fr_t* sh) {
-    const int low = k - s0 - R;
-    const uint32_t G = 1u << gbits;
-    const uint32_t T = 1u << (R + gbits);
-    const uint32_t lchunks = (1u << low) >> gbits;
-    const uint32_t hi = b / lchunks, lc = b % lchunks;
-    const size_t base = ((size_t)hi << (k - s0)) | ((size_t)lc << gbits);
-    for (uint32_t e = threadIdx.x; e < T; e += NTT_THREADS) {
-        uint32_t m = e >> gbits, g = e & (G - 1);
-        sh[e] = x[base + ((size_t)m << low) + g];
+// one pass of one transform over its tiles [tile_begin, next tile_begin)
+struct NttPassDesc {
+    fr_t* x;
+    const fr_t* tw;     // w^e (or w^-e), e < n/2
+    const fr_t* pre;    // x[j] *= pre[j] as the pass loads (first pass only), or null
+    const fr_t* post;   // x[j] *= post[j] as the pass stores (last pass only), or null
+    int k, s0, R, gbits, dit;
+    uint64_t tile_begin;
+};
+
+// LDS slot of tile element e: two 16-byte planes (low and high half of the
+// element), so lanes reading consecutive elements touch consecutive 16-byte
+// words; the low 3 bits are XOR-ed with the next 3 so that threads holding
+// 8 consecutive elements (the lowest radix-8 group) spread over the banks
+HZK_DEV uint32_t lds_slot(uint32_t e) { return e ^ ((e >> 3) & 7); }
+HZK_DEV void lds_put(uint4* sh, uint32_t e, const fr_t& v) {
+    const uint32_t s = lds_slot(e);
+    sh[s] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
+    sh[s + (1u << NTT_TILE_BITS)] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+}
+HZK_DEV fr_t lds_get(const uint4* sh, uint32_t e) {
+    const uint32_t s = lds_slot(e);
+    const uint4 a = sh[s], b = sh[s + (1u << NTT_TILE_BITS)];
+    fr_t v;
+    v.l[0] = a.x, v.l[1] = a.y, v.l[2] = a.z, v.l[3] = a.w, v.l[4] = b.x, v.l[5] = b.y, v.l[6] = b.z, v.l[7] = b.w;
+    return v;
+}
+
+// Geometry of one tile of a pass (stages s0 .. s0+R-1 of a 2^k transform).
+// Tile element e has row m = e >> gbits (the butterfly dimension) and column
+// g = e & (2^gbits - 1); its global index is base + (m << low) + g. The DIF
+// stage s0 + st pairs rows differing in bit R - 1 - st, i.e. tile bit
+// gbits + R - 1 - st, with twiddle w^((((m & mask) << low) | column) << s).
+struct NttTile {
+    int k, s0, R, gbits, low, TB;
+    uint32_t colbase;
+    size_t base;
+    HZK_DEV NttTile(int k_, int s0_, int R_, int gbits_, uint32_t b) : k(k_), s0(s0_), R(R_), gbits(gbits_) {
+        low = k - s0 - R;
+        TB = R + gbits;
+        const uint32_t lchunks = (1u << low) >> gbits;
+        const uint32_t hi = b / lchunks, lc = b % lchunks;
+        base = ((size_t)hi << (k - s0)) | ((size_t)lc << gbits);
+        colbase = lc << gbits;
     }
-    __syncthreads();
-    for (int st = 0; st < R; ++st) {
-        const int s = s0 + st;
+    HZK_DEV size_t gidx(uint32_t e) const { return base + ((size_t)(e >> gbits) << low) + (e & ((1u << gbits) - 1)); }
+    // twiddle exponent of the butterfly of local stage st whose lower element is e
+    HZK_DEV size_t texp(uint32_t e, int st) const {
         const int mp = R - 1 - st;
-        const uint32_t mmask = (1u << mp) - 1;
-        for (uint32_t bf = threadIdx.x; bf < T / 2; bf += NTT_THREADS) {
-            uint32_t g = bf & (G - 1), q = bf >> gbits;
-            uint32_t m0 = ((q >> mp) << (mp + 1)) | (q & mmask);
-            uint32_t i0 = (m0 << gbits) | g, i1 = ((m0 | (1u << mp)) << gbits) | g;
-            size_t e = ((((size_t)(m0 & mmask)) << low) | ((size_t)lc << gbits | g)) << s;
-            fr_t a = sh[i0], c = sh[i1];
-            sh[i0] = a + c;
-            sh[i1] = (a - c) * tw[e];
-        }
-        __syncthreads();
+        const uint32_t m = (e >> gbits) & ((1u << mp) - 1);
+        return ((((size_t)m) << low) | colbase | (e & ((1u << gbits) - 1))) << (s0 + st);
     }
-    for (uint32_t e = threadIdx.x; e < T; e += NTT_THREADS) {
-        uint32_t m = e >> gbits, g = e & (G - 1);
-        x[base + ((size_t)m << low) + g] = sh[e];
+};
+
+// the four butterflies of one stage over the thread's 8 elements v[i]
+// (element i at tile index eb | (i << L)); the stage pairs bit IB of i
+template <int IB, bool DIT>
+HZK_DEV void ntt_stage8(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st) {
+    constexpr int P[4] = {0, 1, 2, 3};
+    int lo[4], hi[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int below = P[j] & ((1 << IB) - 1), above = (P[j] >> IB) << (IB + 1);
+        lo[j] = above | below;
+        hi[j] = lo[j] | (1 << IB);
+    }
+    fr_t t[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[j] = tw[T.texp(eb | ((uint32_t)lo[j] << L), st)];
+    if (!DIT) {
+        fr_t d[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            d[j] = v[lo[j]] - v[hi[j]];
+            v[lo[j]] = v[lo[j]] + v[hi[j]];
+        }
+        fr_t::mul2(v[hi[0]], d[0], t[0], v[hi[1]], d[1], t[1]);
+        fr_t::mul2(v[hi[2]], d[2], t[2], v[hi[3]], d[3], t[3]);
+    } else {
+        fr_t p[4];
+        fr_t::mul2(p[0], v[hi[0]], t[0], p[1], v[hi[1]], t[1]);
+        fr_t::mul2(p[2], v[hi[2]], t[2], p[3], v[hi[3]], t[3]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[hi[j]] = v[lo[j]] - p[j];
+            v[lo[j]] = v[lo[j]] + p[j];
+        }
     }
 }
 
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k, int s0,
-                                                          int R, int gbits) {
-    extern __shared__ fr_t sh[];
-    ntt_tile(x, tw, k, s0, R, gbits, blockIdx.x, sh);
+template <bool DIT>
+HZK_DEV void ntt_stage8_rt(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st, int ib) {
+    if (ib == 0) ntt_stage8<0, DIT>(v, T, tw, eb, L, st);
+    else if (ib == 1) ntt_stage8<1, DIT>(v, T, tw, eb, L, st);
+    else ntt_stage8<2, DIT>(v, T, tw, eb, L, st);
+}
+
+// x[i] *= tab[i] for the thread's 8 elements, interleaved in pairs
+HZK_DEV void mul_table8(fr_t* v, const NttTile& T, const fr_t* __restrict__ tab, uint32_t eb, int L) {
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        const fr_t a = tab[T.gidx(eb | ((uint32_t)i << L))], b = tab[T.gidx(eb | ((uint32_t)(i + 1) << L))];
+        fr_t::mul2(v[i], v[i], a, v[i + 1], v[i + 1], b);
+    }
+}
+
+// One tile of a pass: stages in chunks of up to three. DIF runs local stages
+// 0..R-1 from the first chunk (tile bits TB-1..) down; DIT runs them in
+// reverse. A thread's 8 elements for a chunk span the three tile bits
+// [L, L + 3) that contain the chunk's stage bits; its other tile bits come
+// from the thread index (low bits first, so a wave's lanes read consecutive
+// elements wherever L > 0).
+template <bool DIT>
+HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const fr_t* __restrict__ pre,
+                      const fr_t* __restrict__ post, int k, int s0, int R, int gbits, uint32_t b, uint4* sh) {
+    const NttTile T(k, s0, R, gbits, b);
+    const uint32_t t = threadIdx.x;
+    const bool active = t < (1u << (T.TB - 3));
+    fr_t v[8];
+    const int nch = (R + 2) / 3;
+    uint32_t eb_prev = 0;
+    int L_prev = 0;
+    for (int c = 0; c < nch; ++c) {
+        int stA, len;
+        if (!DIT) {
+            stA = 3 * c;
+            len = min(3, R - stA);
+        } else {
+            stA = max(0, R - 3 * (c + 1));
+            len = R - 3 * c - stA;
+        }
+        const int btop = gbits + R - 1 - stA, blo = btop - len + 1;
+        const int L = min(blo, T.TB - 3);
+        const uint32_t eb = (t & ((1u << L) - 1)) | ((t >> L) << (L + 3));
+        if (c == 0) {
+            if (active) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = x[T.gidx(eb | ((uint32_t)i << L))];
+                if (pre) mul_table8(v, T, pre, eb, L);
+            }
+        } else {
+            if (c > 1) __syncthreads();   // every thread has read the previous exchange
+            if (active)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) lds_put(sh, eb_prev | ((uint32_t)i << L_prev), v[i]);
+            __syncthreads();
+            if (active)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) v[i] = lds_get(sh, eb | ((uint32_t)i << L));
+        }
+        if (active) {
+            if (!DIT)
+                for (int st = stA; st < stA + len; ++st) ntt_stage8_rt<false>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
+            else
+                for (int st = stA + len - 1; st >= stA; --st) ntt_stage8_rt<true>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
+        }
+        eb_prev = eb;
+        L_prev = L;
+    }
+    if (active) {
+        if (post) mul_table8(v, T, post, eb_prev, L_prev);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[T.gidx(eb_prev | ((uint32_t)i << L_prev))] = v[i];
+    }
+}
+
+template <bool DIT>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw,
+                                                          const fr_t* __restrict__ pre, const fr_t* __restrict__ post, int k,
+                                                          int s0, int R, int gbits) {
+    __shared__ uint4 sh[2 << NTT_TILE_BITS];
+    ntt_tile<DIT>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
+}
+
+// transforms of 2 and 4 points (too small for an 8-element group): one thread each
+__global__ void k_ntt_tiny(fr_t* __restrict__ x, const fr_t* __restrict__ tw, int k) {
+    if (threadIdx.x != 0) return;
+    const int n = 1 << k;
+    for (int s = 0; s < k; ++s) {
+        const int h = n >> (s + 1);
+        for (int j = 0; j < n; ++j) {
+            if (j & h) continue;
+            const fr_t a = x[j], c = x[j + h];
+            x[j] = a + c;
+            x[j + h] = (a - c) * tw[(j & (h - 1)) << s];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ batched transforms
-// one (transform, pass) of a round: its tiles occupy [tile_begin, next tile_begin)
-struct NttPassDesc {
-    fr_t* x;
-    const fr_t* tw;
-    int k, s0, R, gbits;
-    uint64_t tile_begin;
-};
 // one transform for the element-wise stages: its elements occupy [begin, next begin)
 struct NttElemDesc {
     fr_t* x;
     const fr_t* factor;   // bit-reverse scale (1/n for inverses) or null
-    const fr_t* pre;      // coset powers applied before the passes, or null
     const fr_t* post;     // coset powers applied after the permutation, or null
     int k;
     uint64_t begin;
@@ -110,19 +264,12 @@ HZK_DEV uint32_t find_elem(const NttElemDesc* d, uint32_t nd, uint64_t gid) {
 }
 
 __global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
-    extern __shared__ fr_t sh[];
-    uint32_t i = find_desc(pd, npd, blockIdx.x);
+    __shared__ uint4 sh[2 << NTT_TILE_BITS];
+    const uint32_t i = find_desc(pd, npd, blockIdx.x);
     const NttPassDesc& d = pd[i];
-    ntt_tile(d.x, d.tw, d.k, d.s0, d.R, d.gbits, (uint32_t)(blockIdx.x - d.tile_begin), sh);
-}
-
-__global__ void __launch_bounds__(256) k_ntt_pre_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
-    uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
-    const NttElemDesc& d = ed[find_elem(ed, ne, gid)];
-    if (!d.pre) return;
-    uint64_t j = gid - d.begin;
-    d.x[j] = d.x[j] * d.pre[j];
+    const uint32_t b = (uint32_t)(blockIdx.x - d.tile_begin);
+    if (d.dit) ntt_tile<true>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    else ntt_tile<false>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
 }
 
 __global__ void __launch_bounds__(256) k_bitrev_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
@@ -172,6 +319,15 @@ __global__ void __launch_bounds__(256) k_mul_table(fr_t* __restrict__ x, const f
     x[i] = v;
 }
 
+// out[p] = tab[rev_k(p)] * scale: a bit-reversed copy of a coset table
+__global__ void __launch_bounds__(256) k_bitrev_table(fr_t* __restrict__ out, const fr_t* __restrict__ tab, int k,
+                                                      const fr_t* __restrict__ scale) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ((size_t)1 << k)) return;
+    size_t r = k ? (size_t)(__brev((uint32_t)p) >> (32 - k)) : 0;
+    out[p] = tab[r] * *scale;
+}
+
 // out[e] = w^e for e < n, from sq[b] = w^(2^b)
 __global__ void __launch_bounds__(256) k_powers(fr_t* __restrict__ out, size_t n, const fr_t* __restrict__ sq) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,10 +358,13 @@ static host::Fr root_of_unity(int k) {
 }
 
 // per transform size: forward / inverse twiddles (n/2 each), coset powers
-// g^j and g^-j (n each) and 1/n; built on first use, kept for the context
+// g^j and g^-j (n each), 1/n, and (built when the Groth16 H pipeline first
+// asks) the bit-reversed coset tables g^rev(p) / n and g^-rev(p) / n; built
+// on first use, kept for the context
 struct NttTables {
     fr_t* tw[2] = {nullptr, nullptr};
     fr_t* coset[2] = {nullptr, nullptr};
+    fr_t* coset_rev[2] = {nullptr, nullptr};
     fr_t* ninv = nullptr;
 };
 struct NttCache {
@@ -221,7 +380,7 @@ void ntt_release(hzk_ctx* ctx) {
     if (!ctx->ntt_cache) return;
     NttCache& c = *(NttCache*)ctx->ntt_cache;
     for (auto& t : c.t)
-        for (auto p : {t.tw[0], t.tw[1], t.coset[0], t.coset[1], t.ninv})
+        for (auto p : {t.tw[0], t.tw[1], t.coset[0], t.coset[1], t.coset_rev[0], t.coset_rev[1], t.ninv})
             if (p) (void)hipFree(p);
     delete &c;
     ctx->ntt_cache = nullptr;
@@ -264,50 +423,16 @@ static int ensure_tables(hzk_ctx* ctx, int logn, NttTables** out) {
     return 0;
 }
 
-static void set_ntt_lds() {
-    static std::once_flag once;
-    std::call_once(once, [] {
-        (void)hipFuncSetAttribute((const void*)k_ntt_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (1 << NTT_TILE_BITS) * 32);
-        (void)hipFuncSetAttribute((const void*)k_ntt_pass_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (1 << NTT_TILE_BITS) * 32);
-    });
-}
-
-// In-place transform with twiddle table tw (w^e, e < n/2), optional device factor.
-int ntt_run(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, const fr_t* factor) {
-    size_t n = (size_t)1 << k;
-    hipStream_t st = ctx->stream;
-    set_ntt_lds();
-    if (k == 0) {
-        if (factor) hipLaunchKernelGGL(k_bitrev_scale, dim3(1), dim3(1), 0, st, x, n, 1, factor);   // 1 x 1: scale only
-        return 0;
+static int ensure_rev_tables(hzk_ctx* ctx, int logn, NttTables* t) {
+    if (t->coset_rev[0]) return 0;
+    const size_t n = (size_t)1 << logn;
+    for (int d = 0; d < 2; ++d) {
+        HZK_HIP(ctx, hipMalloc(&t->coset_rev[d], n * 32));
+        hipLaunchKernelGGL(k_bitrev_table, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, t->coset_rev[d],
+                           (const fr_t*)t->coset[d], logn, (const fr_t*)t->ninv);
+        HZK_HIP(ctx, hipGetLastError());
     }
-    // pass plan: the last pass takes the lowest min(k, 11) stages (contiguous
-    // rows); the rest split evenly into passes of <= 9 stages (>= 4 columns)
-    int last = k < NTT_TILE_BITS ? k : NTT_TILE_BITS;
-    int rest = k - last;
-    int npass = (rest + 8) / 9;
-    int s0 = 0;
-    {
-        TimedLaunch tl(ctx, "ntt_pass");
-        for (int p = 0; p < npass; ++p) {
-            int R = rest / npass + (p < rest % npass ? 1 : 0);
-            int gbits = NTT_TILE_BITS - R;
-            int low = k - s0 - R;
-            if (gbits > low) gbits = low;
-            size_t tiles = n >> (R + gbits);
-            hipLaunchKernelGGL(k_ntt_pass, dim3((unsigned)tiles), dim3(NTT_THREADS), (32u << (R + gbits)), st, x, tw, k, s0, R,
-                               gbits);
-            s0 += R;
-        }
-        hipLaunchKernelGGL(k_ntt_pass, dim3((unsigned)(n >> last)), dim3(NTT_THREADS), (32u << last), st, x, tw, k, s0, last, 0);
-    }
-    {
-        TimedLaunch tl(ctx, "ntt_bitrev");
-        hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, k, factor);
-    }
-    HZK_HIP(ctx, hipGetLastError());
+    HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
@@ -315,6 +440,71 @@ static int log2_exact(size_t n) {
     int k = 0;
     while (((size_t)1 << k) < n) ++k;
     return ((size_t)1 << k) == n ? k : -1;
+}
+
+// Pass plan of a 2^k transform in DIF order, (s0, R, gbits) per pass: the
+// lowest min(k, 11) stages (contiguous rows) form the last pass, the rest
+// split evenly into passes of <= 9 stages (>= 4 columns per tile). A DIT run
+// executes the same passes in reverse order. k >= 3.
+static void plan_passes(int k, std::vector<int3>& out) {
+    out.clear();
+    int last = k < NTT_TILE_BITS ? k : NTT_TILE_BITS;
+    int rest = k - last;
+    int npass = (rest + 8) / 9;
+    int s0 = 0;
+    for (int p = 0; p < npass; ++p) {
+        int R = rest / npass + (p < rest % npass ? 1 : 0);
+        int gbits = NTT_TILE_BITS - R;
+        int low = k - s0 - R;
+        if (gbits > low) gbits = low;
+        out.push_back(make_int3(s0, R, gbits));
+        s0 += R;
+    }
+    out.push_back(make_int3(s0, last, 0));
+}
+
+// The passes of one transform on ctx's stream: DIF (natural -> bit-reversed)
+// or DIT (bit-reversed -> natural), pre applied by the first pass executed,
+// post by the last. k >= 3.
+static void ntt_passes(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, int dit, const fr_t* pre, const fr_t* post) {
+    std::vector<int3> plan;
+    plan_passes(k, plan);
+    if (dit) std::reverse(plan.begin(), plan.end());
+    for (size_t p = 0; p < plan.size(); ++p) {
+        const int3 pl = plan[p];
+        const unsigned tiles = (unsigned)(((size_t)1 << k) >> (pl.y + pl.z));
+        const fr_t* pp = p == 0 ? pre : nullptr;
+        const fr_t* qq = p + 1 == plan.size() ? post : nullptr;
+        if (dit)
+            hipLaunchKernelGGL(k_ntt_pass<true>, dim3(tiles), dim3(NTT_THREADS), 0, ctx->stream, x, tw, pp, qq, k, pl.x,
+                               pl.y, pl.z);
+        else
+            hipLaunchKernelGGL(k_ntt_pass<false>, dim3(tiles), dim3(NTT_THREADS), 0, ctx->stream, x, tw, pp, qq, k, pl.x,
+                               pl.y, pl.z);
+    }
+}
+
+// In-place transform with twiddle table tw (w^e, e < n/2) and the
+// reference's natural-in, natural-out contract; optional device factor.
+int ntt_run(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, const fr_t* factor) {
+    size_t n = (size_t)1 << k;
+    hipStream_t st = ctx->stream;
+    if (k == 0) {
+        if (factor) hipLaunchKernelGGL(k_bitrev_scale, dim3(1), dim3(1), 0, st, x, n, 1, factor);   // 1 x 1: scale only
+        HZK_HIP(ctx, hipGetLastError());
+        return 0;
+    }
+    {
+        TimedLaunch tl(ctx, "ntt_pass");
+        if (k < 3) hipLaunchKernelGGL(k_ntt_tiny, dim3(1), dim3(64), 0, st, x, tw, k);
+        else ntt_passes(ctx, x, k, tw, 0, nullptr, nullptr);
+    }
+    {
+        TimedLaunch tl(ctx, "ntt_bitrev");
+        hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, k, factor);
+    }
+    HZK_HIP(ctx, hipGetLastError());
+    return 0;
 }
 
 int ntt_standard(hzk_ctx* ctx, fr_t* x, size_t deg, int inverse) {
@@ -335,6 +525,17 @@ int coset_ntt(hzk_ctx* ctx, fr_t* x, size_t deg, int inverse) {
     int rc = ensure_tables(ctx, k, &t);
     if (rc) return rc;
     if (!inverse) {
+        if (k >= 3) {   // the coset powers ride on the first pass's loads
+            {
+                TimedLaunch tl(ctx, "ntt_pass");
+                ntt_passes(ctx, x, k, t->tw[0], 0, t->coset[0], nullptr);
+            }
+            TimedLaunch tl(ctx, "ntt_bitrev");
+            hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(deg, 256)), dim3(256), 0, ctx->stream, x, deg, k,
+                               (const fr_t*)nullptr);
+            HZK_HIP(ctx, hipGetLastError());
+            return 0;
+        }
         {
             TimedLaunch tl(ctx, "ntt_coset_scale");
             hipLaunchKernelGGL(k_mul_table, dim3(blocks_for(deg, 256)), dim3(256), 0, ctx->stream, x, (const fr_t*)t->coset[0],
@@ -351,30 +552,15 @@ int coset_ntt(hzk_ctx* ctx, fr_t* x, size_t deg, int inverse) {
     return 0;
 }
 
-// pass plan of ntt_run: (s0, R, gbits) per pass, the 11-stage contiguous pass last
-static void plan_passes(int k, std::vector<int3>& out) {
-    out.clear();
-    if (k == 0) return;
-    int last = k < NTT_TILE_BITS ? k : NTT_TILE_BITS;
-    int rest = k - last;
-    int npass = (rest + 8) / 9;
-    int s0 = 0;
-    for (int p = 0; p < npass; ++p) {
-        int R = rest / npass + (p < rest % npass ? 1 : 0);
-        int gbits = NTT_TILE_BITS - R;
-        int low = k - s0 - R;
-        if (gbits > low) gbits = low;
-        out.push_back(make_int3(s0, R, gbits));
-        s0 += R;
-    }
-    out.push_back(make_int3(s0, last, 0));
-}
-
-// Batched transforms: kind 0 NTT, 1 inverse NTT (scaled 1/n), 2 coset NTT,
-// 3 coset inverse NTT. Passes are right-aligned into rounds so every
-// transform's last (contiguous) pass runs in the final round; each round is one
-// launch over the tiles of every transform in it, then one launch applies the
-// bit-reverse permutation with the per-transform scale / coset factors.
+// Batched transforms. kind 0 NTT, 1 inverse NTT (scaled 1/n), 2 coset NTT,
+// 3 coset inverse NTT, all natural order in and out; and, for the Groth16
+// H pipeline (no permutation): 4 inverse NTT, natural in, bit-reversed out,
+// unscaled; 5 coset NTT of bit-reversed coefficients c scaled by 1/n on the
+// way in (x[p] * g^rev(p) / n), natural out; 6 coset inverse NTT, natural
+// in, bit-reversed out, with g^-rev(p) / n applied on the way out. Pass p of
+// every transform runs in round p, each round one launch over the tiles of
+// every transform in it; then one launch applies the bit-reverse
+// permutation with the per-transform scale / coset factors to kinds 0..3.
 int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     HZK_CHECK(ctx, jobs && count > 0, HZK_ERR_INVALID, "ntt batch: empty");
     std::vector<int> ks(count);
@@ -382,46 +568,50 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     std::vector<std::vector<int3>> plans(count);
     size_t rounds = 0;
     for (size_t i = 0; i < count; ++i) {
-        HZK_CHECK(ctx, jobs[i].x && jobs[i].kind >= 0 && jobs[i].kind <= 3, HZK_ERR_INVALID, "ntt batch: bad job %zu", i);
+        HZK_CHECK(ctx, jobs[i].x && jobs[i].kind >= 0 && jobs[i].kind <= 6, HZK_ERR_INVALID, "ntt batch: bad job %zu", i);
         int k = log2_exact(jobs[i].deg);
         HZK_CHECK(ctx, k >= 0 && k <= NTT_MAX_LOG, HZK_ERR_INVALID, "ntt batch: deg %zu of job %zu is not a power of two <= 2^%d",
                   jobs[i].deg, i, NTT_MAX_LOG);
+        HZK_CHECK(ctx, jobs[i].kind <= 3 || k >= 3, HZK_ERR_INVALID, "ntt batch: kind %d needs deg >= 8", jobs[i].kind);
         ks[i] = k;
         int rc = ensure_tables(ctx, k, &tabs[i]);
+        if (!rc && jobs[i].kind >= 5) rc = ensure_rev_tables(ctx, k, tabs[i]);
         if (rc) return rc;
-        plan_passes(k, plans[i]);
+        if (k >= 3) {
+            plan_passes(k, plans[i]);
+            if (jobs[i].kind == 5) std::reverse(plans[i].begin(), plans[i].end());
+        }
         rounds = std::max(rounds, plans[i].size());
     }
-    set_ntt_lds();
     // pass descriptors, round by round
     std::vector<NttPassDesc> pds;
     std::vector<size_t> round_begin(rounds + 1, 0);
     std::vector<uint64_t> round_tiles(rounds, 0);
-    std::vector<uint32_t> round_lds(rounds, 0);
     for (size_t r = 0; r < rounds; ++r) {
         round_begin[r] = pds.size();
         uint64_t tiles = 0;
         for (size_t i = 0; i < count; ++i) {
-            size_t m = plans[i].size();
-            if (m + r < rounds) continue;
-            int3 pl = plans[i][m + r - rounds];
-            int inv = jobs[i].kind & 1;
-            NttPassDesc d{(fr_t*)jobs[i].x, tabs[i]->tw[inv], ks[i], pl.x, pl.y, pl.z, tiles};
+            const size_t m = plans[i].size();
+            if (r >= m) continue;
+            const int3 pl = plans[i][r];
+            const int kind = jobs[i].kind;
+            const int inv = kind == 1 || kind == 3 || kind == 4 || kind == 6;
+            const fr_t* pre = r == 0 ? (kind == 2 ? tabs[i]->coset[0] : kind == 5 ? tabs[i]->coset_rev[0] : nullptr) : nullptr;
+            const fr_t* post = r + 1 == m && kind == 6 ? tabs[i]->coset_rev[1] : nullptr;
+            NttPassDesc d{(fr_t*)jobs[i].x, tabs[i]->tw[inv], pre, post, ks[i], pl.x, pl.y, pl.z, kind == 5, tiles};
             pds.push_back(d);
             tiles += (uint64_t)1 << (ks[i] - pl.y - pl.z);
-            round_lds[r] = std::max(round_lds[r], 32u << (pl.y + pl.z));
         }
         round_tiles[r] = tiles;
     }
     round_begin[rounds] = pds.size();
-    std::vector<NttElemDesc> eds(count);
+    std::vector<NttElemDesc> eds;
     uint64_t elems = 0;
-    bool any_pre = false;
     for (size_t i = 0; i < count; ++i) {
-        int kind = jobs[i].kind;
-        eds[i] = {(fr_t*)jobs[i].x, (kind & 1) ? tabs[i]->ninv : nullptr, kind == 2 ? tabs[i]->coset[0] : nullptr,
-                  kind == 3 ? tabs[i]->coset[1] : nullptr, ks[i], elems};
-        any_pre |= kind == 2;
+        const int kind = jobs[i].kind;
+        if (kind > 3) continue;
+        eds.push_back({(fr_t*)jobs[i].x, (kind & 1) ? tabs[i]->ninv : nullptr, kind == 3 ? tabs[i]->coset[1] : nullptr, ks[i],
+                       elems});
         elems += jobs[i].deg;
     }
     // descriptors to the device
@@ -448,18 +638,28 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     const NttElemDesc* ded = (const NttElemDesc*)((char*)ctx->desc + eoff);
 
     auto stages = [&]() {
-        if (any_pre)
-            hipLaunchKernelGGL(k_ntt_pre_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)count, elems);
+        for (size_t i = 0; i < count; ++i) {   // 2- and 4-point transforms
+            if (ks[i] == 0 || ks[i] >= 3) continue;
+            const int kind = jobs[i].kind, inv = kind & 1;
+            if (kind == 2)
+                hipLaunchKernelGGL(k_mul_table, dim3(1), dim3(256), 0, st, (fr_t*)jobs[i].x, (const fr_t*)tabs[i]->coset[0],
+                                   jobs[i].deg, (const fr_t*)nullptr);
+            hipLaunchKernelGGL(k_ntt_tiny, dim3(1), dim3(64), 0, st, (fr_t*)jobs[i].x, (const fr_t*)tabs[i]->tw[inv], ks[i]);
+        }
         for (size_t r = 0; r < rounds; ++r) {
             uint32_t np = (uint32_t)(round_begin[r + 1] - round_begin[r]);
-            hipLaunchKernelGGL(k_ntt_pass_batch, dim3((unsigned)round_tiles[r]), dim3(NTT_THREADS), round_lds[r], st,
+            hipLaunchKernelGGL(k_ntt_pass_batch, dim3((unsigned)round_tiles[r]), dim3(NTT_THREADS), 0, st,
                                dpd + round_begin[r], np);
         }
-        hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)count, elems);
+        if (elems)
+            hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)eds.size(), elems);
     };
     if (capture) {
         std::string key = "ntt:";
-        for (size_t i = 0; i < count; ++i) key += std::to_string(ks[i]) + "/" + std::to_string(jobs[i].kind) + ";";
+        for (size_t i = 0; i < count; ++i) {
+            key += std::to_string(ks[i]) + "/" + std::to_string(jobs[i].kind) + ";";
+            if (ks[i] > 0 && ks[i] < 3) key += std::to_string((uintptr_t)jobs[i].x) + ";";   // launched with direct arguments
+        }
         hzk_graph_entry& g = ctx->graphs[key];
         if (!g.exec || g.generation != ctx->desc_generation) {
             if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
