@@ -1,0 +1,254 @@
+// Fq in nine 29-bit limbs for the G1 bucket accumulation (the MSM's hot loop).
+//
+// With 32-bit limbs every 32x32 partial product of a Montgomery product is a
+// v_mad_u64_u32 plus a v_addc_co_u32 that folds its carry-out: two VALU
+// instructions per partial product. With 29-bit limbs a 64-bit column
+// accumulator cannot overflow (a column holds at most 9 products of two
+// limbs below 2^30, 9 reduction products below 2^58 and the previous
+// column's carry: < 2^63.5), so each partial product is one v_mad_u64_u32
+// into the accumulator and nothing else: 162 multiply-adds per product
+// instead of 128 multiply-adds and 128 carry folds.
+//
+// Montgomery form with R = 2^261 (nine full 29-bit reduction digits). Values
+// are lazily reduced: a product returns a value below 2p whenever the two
+// inputs' bounds, in units of p, multiply to less than 2^261 / p > 168 (the
+// caller's invariant, checked in the point formulas' comments); sums and
+// differences carry explicit multiples of p. The bases and the bucket sums
+// stay in the library's R = 2^256 form in memory: from_r256 / to_r256
+// convert with one product each.
+#pragma once
+#include "curve.hpp"
+#include "field.hpp"
+
+namespace hzk {
+
+struct fq29 {
+    static constexpr uint32_t MASK = (1u << 29) - 1;
+    static constexpr uint32_t P[9] = {0x187cfd47, 0x010460b6, 0x1c72a34f, 0x02d522d0, 0x1585d978,
+                                      0x02db40c0, 0x00a6e141, 0x0e5c2634, 0x0030644e};
+    static constexpr uint32_t M0 = 0x04866389;     // -p^-1 mod 2^29
+    static constexpr uint32_t PINV = 0x1b799c77;   // p^-1 mod 2^29
+    static constexpr uint32_t ONE[9] = {0x157ccc21, 0x141c2758, 0x185230d3, 0x014c0419, 0x0aa36fb9,
+                                        0x1d4240ce, 0x11d54c07, 0x052ac7a8, 0x000dc836};   // 2^261 mod p
+    static constexpr uint32_t TO[9] = {0x13349ca1, 0x1a5d84a8, 0x0a3e5cac, 0x100249e0, 0x12b951e8,
+                                       0x0e92d304, 0x14cb95b3, 0x041b9d3d, 0x00058003};    // 2^266 mod p
+    static constexpr uint32_t FROM[9] = {0x058f0d9d, 0x1aea1c6e, 0x11c2cf74, 0x11d651eb, 0x1462c0a7,
+                                         0x11b7bc3c, 0x1cbd99ba, 0x183340fb, 0x000e0a77};  // 2^256 mod p
+    // k p for the biases of subtractions
+    static constexpr uint32_t P1[9] = {0x187cfd47, 0x010460b6, 0x1c72a34f, 0x02d522d0, 0x1585d978,
+                                       0x02db40c0, 0x00a6e141, 0x0e5c2634, 0x0030644e};
+    static constexpr uint32_t P2[9] = {0x10f9fa8e, 0x0208c16d, 0x18e5469e, 0x05aa45a1, 0x0b0bb2f0,
+                                       0x05b68181, 0x014dc282, 0x1cb84c68, 0x0060c89c};
+    static constexpr uint32_t P4[9] = {0x01f3f51c, 0x041182db, 0x11ca8d3c, 0x0b548b43, 0x161765e0,
+                                       0x0b6d0302, 0x029b8504, 0x197098d0, 0x00c19139};
+    static constexpr uint32_t P6[9] = {0x12edefaa, 0x061a4448, 0x0aafd3da, 0x10fed0e5, 0x012318d0,
+                                       0x11238484, 0x03e94786, 0x1628e538, 0x012259d6};
+    static constexpr uint32_t P8[9] = {0x03e7ea38, 0x082305b6, 0x03951a78, 0x16a91687, 0x0c2ecbc0,
+                                       0x16da0605, 0x05370a08, 0x12e131a0, 0x01832273};
+
+    uint32_t l[9];
+
+    static HZK_DEV fq29 zero() { fq29 r; for (int i = 0; i < 9; ++i) r.l[i] = 0; return r; }
+    static HZK_DEV fq29 one() { fq29 r; for (int i = 0; i < 9; ++i) r.l[i] = ONE[i]; return r; }
+    static HZK_DEV fq29 konst(const uint32_t* c) { fq29 r; for (int i = 0; i < 9; ++i) r.l[i] = c[i]; return r; }
+
+    // repack 8 x 32-bit limbs (a value < 2^256) into 9 x 29-bit limbs
+    static HZK_DEV fq29 unpack(const uint32_t* w) {
+        fq29 r;
+        r.l[0] = w[0] & MASK;
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            const int b = 29 * i, q = b >> 5, s = b & 31;   // limb i starts at bit s of word q
+            r.l[i] = (uint32_t)(((((uint64_t)w[q + 1]) << 32) | w[q]) >> s) & MASK;
+        }
+        r.l[8] = w[7] >> 8;   // bits 232..255
+        return r;
+    }
+    // normalized limbs of a value < 2^256 back to 8 x 32-bit limbs
+    HZK_DEV void pack(uint32_t* w) const {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int b = 32 * q, i = b / 29, s = b % 29;   // word q starts at bit s of limb i
+            uint64_t v = (uint64_t)l[i] >> s;
+            v |= (uint64_t)l[i + 1] << (29 - s);
+            if (i + 2 < 9 && 58 - s < 32) v |= (uint64_t)l[i + 2] << (58 - s);
+            w[q] = (uint32_t)v;
+        }
+    }
+
+    // a + b limb by limb, no carries: limbs below 2^30 for normalized inputs
+    friend HZK_DEV fq29 operator+(const fq29& a, const fq29& b) {
+        fq29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = a.l[i] + b.l[i];
+        return r;
+    }
+    // a - b + K (K = kp, kp >= the value of b), normalized; limbs of a and b
+    // below 2^31, values such that the result is below 2^261
+    static HZK_DEV fq29 sub(const fq29& a, const fq29& b, const uint32_t* K) {
+        fq29 r;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int32_t v = (int32_t)(a.l[i] + K[i] - b.l[i]) + c;
+            r.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+    // kp - a for a value below kp, normalized
+    static HZK_DEV fq29 neg(const fq29& a, const uint32_t* K) { return sub(zero(), a, K); }
+    // carries propagated: normalized limbs of the same value (limbs below 2^31)
+    HZK_DEV fq29 norm() const {
+        fq29 r;
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const uint32_t v = l[i] + c;
+            r.l[i] = v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+
+    // a b 2^-261 mod p, lazily reduced (see the header); inputs' limbs below 2^30
+    static HZK_DEV void mont_mul(fq29& r, const fq29& a, const fq29& b) {
+        uint32_t m[9];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+#pragma unroll
+            for (int j = 0; j < k; ++j) acc += (uint64_t)m[j] * P[k - j];
+#pragma unroll
+            for (int j = 0; j <= k; ++j) acc += (uint64_t)a.l[j] * b.l[k - j];
+            m[k] = ((uint32_t)acc * M0) & MASK;
+            acc += (uint64_t)m[k] * P[0];
+            acc >>= 29;
+        }
+#pragma unroll
+        for (int k = 9; k < 17; ++k) {
+#pragma unroll
+            for (int j = k - 8; j < 9; ++j) acc += (uint64_t)a.l[j] * b.l[k - j];
+#pragma unroll
+            for (int j = k - 8; j < 9; ++j) acc += (uint64_t)m[j] * P[k - j];
+            r.l[k - 9] = (uint32_t)acc & MASK;
+            acc >>= 29;
+        }
+        r.l[8] = (uint32_t)acc;
+    }
+    friend HZK_DEV fq29 operator*(const fq29& a, const fq29& b) { fq29 r; mont_mul(r, a, b); return r; }
+    HZK_DEV fq29 sqr() const { return *this * *this; }
+
+    // a value below 2^261 reduced into [0, p): subtract q p with q from the
+    // top limb (an underestimate by at most one), then p once more if needed
+    HZK_DEV fq29 canonical() const {
+        // q = floor(l8 / (P[8] + 1)) <= floor(value / p); the remainder is below 2p + ...
+        fq29 r = *this;
+        for (int round = 0; round < 2; ++round) {
+            const uint32_t q = r.l[8] / (P[8] + 1);
+            if (q) {
+                int64_t c = 0;
+                for (int i = 0; i < 9; ++i) {
+                    const int64_t v = (int64_t)r.l[i] - (int64_t)q * P[i] + c;
+                    r.l[i] = (uint32_t)v & MASK;
+                    c = v >> 29;
+                }
+            }
+        }
+        // now below 2p (l8 <= P8 means value < (P8 + 1) 2^232 <= 2p): one conditional subtraction
+        fq29 d;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int32_t v = (int32_t)r.l[i] - (int32_t)P[i] + c;
+            d.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return c < 0 ? r : d;
+    }
+
+    // value == 0 mod p, for a normalized value below 16p: v = k p exactly
+    // when k = v p^-1 mod 2^29 is small and k p equals v
+    HZK_DEV bool is_zero_mod_p() const {
+        const uint32_t k = (l[0] * PINV) & MASK;
+        if (k > 16) return false;
+        if (k == 0) {
+            uint32_t t = 0;
+            for (int i = 0; i < 9; ++i) t |= l[i];
+            return t == 0;
+        }
+        uint64_t c = 0;
+        for (int i = 0; i < 9; ++i) {
+            c += (uint64_t)k * P[i];
+            if (((uint32_t)c & MASK) != l[i]) return false;
+            c >>= 29;
+        }
+        return c == 0;
+    }
+
+    // R = 2^256 form (8 x 32-bit limbs, < 2^256) <-> this form: one product each
+    static HZK_DEV fq29 from_r256(const fq_t& x) { return unpack(x.l) * konst(TO); }
+    HZK_DEV fq_t to_r256() const {
+        const fq29 v = (*this * konst(FROM)).canonical();
+        fq_t r;
+        v.pack(r.l);
+        return r;
+    }
+};
+
+// G1 bucket accumulator over fq29 in XYZZ coordinates (madd-2008-s), with the
+// point at infinity as a flag. Invariants (values in units of p): X < 8,
+// Y < 4, ZZ < 2, ZZZ < 2; affine inputs are below p.
+struct g1acc29 {
+    fq29 X, Y, ZZ, ZZZ;
+    bool inf;
+
+    HZK_DEV void set_inf() { inf = true; }
+    HZK_DEV void set_affine(const fq29& x, const fq29& y) {
+        X = x, Y = y, ZZ = fq29::one(), ZZZ = fq29::one();
+        inf = false;
+    }
+    // 2 (x, y), mdbl-2008-s-1: U = 2y, V = U^2, W = U V, S = x V, M = 3 x^2,
+    // X3 = M^2 - 2S, Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W
+    HZK_DEV void set_dbl_affine(const fq29& x, const fq29& y) {
+        const fq29 U = (y + y).norm();                   // < 2
+        const fq29 V = U * U, xx = x * x;                // 2 x 2, 1 x 1
+        const fq29 W = U * V, S = x * V;                 // 2 x 2, 1 x 2
+        const fq29 M = (xx + xx + xx).norm();            // < 6
+        const fq29 MM = M * M;                           // 6 x 6
+        X = fq29::sub(MM, S + S, fq29::P4);              // < 6
+        const fq29 D = fq29::sub(S, X, fq29::P8);        // < 10
+        const fq29 t = M * D, Wy = W * y;                // 6 x 10, 2 x 1
+        Y = fq29::sub(t, Wy, fq29::P2);                  // < 4
+        ZZ = V, ZZZ = W;
+        inf = false;
+    }
+    // this += (x, y) for a finite affine point (x, y < p), every special case handled
+    HZK_DEV void add_affine(const fq29& x, const fq29& y) {
+        if (inf) { set_affine(x, y); return; }
+        const fq29 U2 = x * ZZ, S2 = y * ZZZ;            // 1 x 2 each
+        const fq29 P = fq29::sub(U2, X, fq29::P8);       // < 10
+        const fq29 R = fq29::sub(S2, Y, fq29::P4);       // < 6
+        if (P.is_zero_mod_p()) {
+            if (R.is_zero_mod_p()) set_dbl_affine(x, y);
+            else set_inf();
+            return;
+        }
+        const fq29 PP = P * P, RR = R * R;               // 10 x 10, 6 x 6
+        const fq29 PPP = P * PP, Q = X * PP;             // 10 x 2, 8 x 2
+        const fq29 ZZ3 = ZZ * PP;                        // 2 x 2
+        X = fq29::sub(RR, PPP + Q + Q, fq29::P6);        // RR - (PPP + 2Q) + 6p < 8
+        const fq29 D = fq29::sub(Q, X, fq29::P8);        // < 10
+        const fq29 t = R * D, YP = Y * PPP;              // 6 x 10, 4 x 2
+        ZZZ = ZZZ * PPP;                                 // 2 x 2
+        Y = fq29::sub(t, YP, fq29::P2);                  // < 4
+        ZZ = ZZ3;
+    }
+    // the R = 2^256 XYZZ point (canonical coordinates; infinity = all zero)
+    HZK_DEV xyzz_t<fq_t> to_xyzz() const {
+        if (inf) return xyzz_t<fq_t>::inf();
+        return {X.to_r256(), Y.to_r256(), ZZ.to_r256(), ZZZ.to_r256()};
+    }
+};
+
+}  // namespace hzk

@@ -9,6 +9,7 @@
 #include <vector>
 #include "common.hpp"
 #include "curve.hpp"
+#include "field29.hpp"
 #include "host_ec.hpp"
 #include "msm.hpp"
 
@@ -103,6 +104,115 @@ __global__ void __launch_bounds__(256, 2) k_msm_accumulate(MsmBatchDev b, const 
                 }
             }
         }
+    }
+}
+
+// G1: the same chunk walk with the additions in 29-bit-limb arithmetic
+// (field29.hpp: one v_mad_u64_u32 per partial product, lazy reduction) over
+// the bases converted to its Montgomery form by k_msm_convert_g1; complete
+// buckets and chunk partials are written back in the R = 2^256 form, so the
+// later stages are shared with G2.
+static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev b, const uint32_t* __restrict__ sorted,
+                                                           const uint32_t* __restrict__ offs, g1_xyzz* __restrict__ bucket,
+                                                           g1_xyzz* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
+    const uint64_t total = cbeg[b.ninst];
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
+        const MsmInst& d = b.inst[ii];
+        const uint32_t E = b.stats[4 * ii];
+        const uint32_t e0 = (uint32_t)(gid - cbeg[ii]) * d.K;
+        if (e0 >= E) continue;
+        const uint32_t e1 = min(e0 + d.K, E);
+        const uint64_t kb = space_g(b, SP_KEY)[ii];
+        const uint32_t* of = offs + kb;
+        uint32_t k = find_key(of, d.nk, e0);
+        chunk2key[gid] = k;
+        uint32_t next = of[k + 1];
+        bool before = of[k] < e0;
+        const uint4* srt4 = (const uint4*)(sorted + space_g(b, SP_ENTRY)[ii]);
+        const uint4* bases = (const uint4*)d.bases;   // converted: x | y, 8 x 32-bit limbs each
+        g1acc29 acc;
+        acc.set_inf();
+        uint4 q = srt4[e0 >> 2];
+        uint32_t idx = q.x;
+        uint4 p0, p1, p2, p3;
+        {
+            const uint4* bp = bases + 4 * (size_t)(idx & 0x7fffffffu);
+            p0 = bp[0], p1 = bp[1], p2 = bp[2], p3 = bp[3];
+        }
+        for (uint32_t e = e0; e < e1; ++e) {
+            const bool neg = idx >> 31;
+            const uint32_t xw[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+            const uint32_t yw[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+            const bool inf = p1.w == 0xffffffffu;   // the converter's mark for the point at infinity
+            if (e + 1 < e1) {   // prefetch the next base while this one is added
+                const uint32_t t = (e + 1) & 3;
+                if (t == 0) q = srt4[(e + 1) >> 2];
+                idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
+                const uint4* bp = bases + 4 * (size_t)(idx & 0x7fffffffu);
+                p0 = bp[0], p1 = bp[1], p2 = bp[2], p3 = bp[3];
+            }
+            if (!inf) {
+                const fq29 x = fq29::unpack(xw), y0 = fq29::unpack(yw);
+                const fq29 y = neg ? fq29::neg(y0, fq29::P1) : y0;
+                acc.add_affine(x, y);
+            }
+            if (e + 1 == next || e + 1 == e1) {
+                const g1_xyzz r = acc.to_xyzz();
+                if (before) carry[2 * gid] = r;
+                else if (next > e1) carry[2 * gid + 1] = r;
+                else bucket[kb + k] = r;
+                if (e + 1 < e1) {   // next non-empty bucket
+                    do { ++k; next = of[k + 1]; } while (next <= e + 1);
+                    before = false;
+                    acc.set_inf();
+                }
+            }
+        }
+    }
+}
+
+// one base array to convert for the G1 accumulation: count affine points
+// from src into dst, as items [begin, next begin) of the conversion space
+struct MsmConvTask {
+    const void* src;
+    void* dst;
+    uint64_t begin;
+};
+
+// x, y -> x 2^5, y 2^5 mod p (the 2^261 Montgomery form of field29.hpp),
+// canonical, 8 x 32-bit limbs; the point at infinity (all zero) becomes
+// x word 7 = 0xffffffff, a value no canonical coordinate has
+static __global__ void __launch_bounds__(256) k_msm_convert_g1(const MsmConvTask* __restrict__ tasks, uint32_t ntasks, uint64_t total) {
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ntasks;   // tasks[lo].begin <= gid < tasks[hi].begin
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tasks[mid].begin <= gid) lo = mid; else hi = mid;
+        }
+        const MsmConvTask& t = tasks[lo];
+        const uint64_t i = gid - t.begin;
+        const uint4* sp = (const uint4*)t.src + 4 * i;
+        uint4* dp = (uint4*)t.dst + 4 * i;
+        const uint4 a = sp[0], b2 = sp[1], c = sp[2], e = sp[3];
+        const uint32_t any = a.x | a.y | a.z | a.w | b2.x | b2.y | b2.z | b2.w | c.x | c.y | c.z | c.w | e.x | e.y | e.z | e.w;
+        if (!any) {
+            dp[0] = make_uint4(0, 0, 0, 0);
+            dp[1] = make_uint4(0, 0, 0, 0xffffffffu);
+            dp[2] = make_uint4(0, 0, 0, 0);
+            dp[3] = make_uint4(0, 0, 0, 0);
+            continue;
+        }
+        const uint32_t xw[8] = {a.x, a.y, a.z, a.w, b2.x, b2.y, b2.z, b2.w};
+        const uint32_t yw[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+        uint32_t xo[8], yo[8];
+        (fq29::unpack(xw) * fq29::konst(fq29::TO)).canonical().pack(xo);
+        (fq29::unpack(yw) * fq29::konst(fq29::TO)).canonical().pack(yo);
+        dp[0] = make_uint4(xo[0], xo[1], xo[2], xo[3]);
+        dp[1] = make_uint4(xo[4], xo[5], xo[6], xo[7]);
+        dp[2] = make_uint4(yo[0], yo[1], yo[2], yo[3]);
+        dp[3] = make_uint4(yo[4], yo[5], yo[6], yo[7]);
     }
 }
 
@@ -253,14 +363,36 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xy
     segsum[2 * gid + 1] = run;   // R_r
 }
 
-// tree sum of sh[0..T) into sh[0]
+// Per-thread point slots in LDS, word-major: word w of slot t at
+// sh[w * T + t], so the lanes of a wave touch consecutive words (a point per
+// slot in a row would put every lane's word w in the same bank: 128- or
+// 256-byte strides)
 template <class F, int T>
-HZK_DEV void lds_tree_sum(xyzz_t<F>* sh, uint32_t t) {
+struct LdsPoints {
+    static constexpr int W = sizeof(xyzz_t<F>) / 4;
+    uint32_t* sh;
+    HZK_DEV void put(uint32_t t, const xyzz_t<F>& p) const {
+        const uint32_t* w = (const uint32_t*)&p;
+#pragma unroll
+        for (int i = 0; i < W; ++i) sh[i * T + t] = w[i];
+    }
+    HZK_DEV xyzz_t<F> get(uint32_t t) const {
+        xyzz_t<F> p;
+        uint32_t* w = (uint32_t*)&p;
+#pragma unroll
+        for (int i = 0; i < W; ++i) w[i] = sh[i * T + t];
+        return p;
+    }
+};
+
+// tree sum of slots 0..T) into slot 0
+template <class F, int T>
+HZK_DEV void lds_tree_sum(const LdsPoints<F, T>& sh, uint32_t t) {
     for (uint32_t off = T / 2; off > 0; off >>= 1) {
         if (t < off) {
-            xyzz_t<F> x = sh[t];
-            x.add(sh[t + off]);
-            sh[t] = x;
+            xyzz_t<F> x = sh.get(t);
+            x.add(sh.get(t + off));
+            sh.put(t, x);
         }
         __syncthreads();
     }
@@ -272,7 +404,8 @@ HZK_DEV void lds_tree_sum(xyzz_t<F>* sh, uint32_t t) {
 template <class F, int T>
 __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
                                                   xyzz_t<F>* __restrict__ winsum) {
-    __shared__ xyzz_t<F> sh[T];
+    __shared__ uint32_t shw[LdsPoints<F, T>::W * T];
+    const LdsPoints<F, T> sh{shw};
     const uint64_t* wbeg = space_g(b, SP_WIN);
     uint32_t ii = find_inst_g(wbeg, b.ninst, blockIdx.x);
     const MsmInst& d = b.inst[ii];
@@ -288,19 +421,19 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>
             run.add(ss[2 * i + 1]);
             if (i > 0) w.add(run);   // w = sum_i i * R_i
         }
-    // inclusive suffix scan of the thread run sums: sh[t] = sum_{t' >= t} rr_t'
-    sh[t] = run;
+    // inclusive suffix scan of the thread run sums: slot t = sum_{t' >= t} rr_t'
+    sh.put(t, run);
     __syncthreads();
     for (uint32_t off = 1; off < T; off <<= 1) {
-        xyzz_t<F> x = sh[t];
-        if (t + off < T) x.add(sh[t + off]);
+        xyzz_t<F> x = sh.get(t);
+        if (t + off < T) x.add(sh.get(t + off));
         __syncthreads();
-        sh[t] = x;
+        sh.put(t, x);
         __syncthreads();
     }
-    const xyzz_t<F> part_sum = sh[0];
-    // P = sum_t a_t + g * sum_t (w_t + m * X_t), X_t = sh[t] for t >= 1
-    xyzz_t<F> u = t ? sh[t] : xyzz_t<F>::inf();
+    const xyzz_t<F> part_sum = sh.get(0);
+    // P = sum_t a_t + g * sum_t (w_t + m * X_t), X_t = slot t for t >= 1
+    xyzz_t<F> u = t ? sh.get(t) : xyzz_t<F>::inf();
     for (uint32_t k = 1; k < m; k <<= 1) u = u.dbl();
     u.add(w);
     if (T == 256) {
@@ -308,27 +441,27 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>
         for (uint32_t k = 1; k < d.g; k <<= 1) u = u.dbl();
         u.add(a);
         __syncthreads();
-        sh[t] = u;
+        sh.put(t, u);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
         if (t == 0) {
-            winsum[2 * blockIdx.x] = sh[0];         // P_p
+            winsum[2 * blockIdx.x] = sh.get(0);     // P_p
             winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
         }
     } else {
         // less work: both sums first, the factor g once
         __syncthreads();
-        sh[t] = u;
+        sh.put(t, u);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
-        xyzz_t<F> total = sh[0];
+        xyzz_t<F> total = sh.get(0);
         __syncthreads();
-        sh[t] = a;
+        sh.put(t, a);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
         if (t == 0) {
             for (uint32_t k = 1; k < d.g; k <<= 1) total = total.dbl();
-            total.add(sh[0]);
+            total.add(sh.get(0));
             winsum[2 * blockIdx.x] = total;         // P_p
             winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
         }
@@ -348,6 +481,12 @@ struct MsmBatchLayout {
     // scratch offsets
     size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_coarse, o_chunk2key, o_bucket,
         o_carry, o_segsum, o_win;
+    // G1: base arrays converted for the accumulation, one per distinct source
+    // array (its largest referenced point count), and which one each instance reads
+    std::vector<MsmConvTask> conv;     // dst holds the offset into the scratch until run time
+    std::vector<uint32_t> conv_of;     // instance -> task
+    uint64_t conv_total = 0;
+    size_t o_conv = 0, o_convdesc = 0;
 };
 
 template <class F>
@@ -461,6 +600,22 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
     L.o_segsum = take(2 * L.tot[SP_BRED] * sizeof(xyzz_t<F>));   // A_r, R_r per run
     L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));       // P_p, RS_p per part
+    if (!g2) {
+        std::map<const void*, uint64_t> need;   // source array -> points referenced (virtual points of level tables)
+        for (const MsmInst& d : L.inst) need[d.bases] = std::max<uint64_t>(need[d.bases], d.n);
+        std::map<const void*, uint32_t> task_of;
+        uint64_t pts = 0;
+        for (const auto& kv : need) {
+            task_of[kv.first] = (uint32_t)L.conv.size();
+            L.conv.push_back({kv.first, (void*)(uintptr_t)(pts * 64), pts});
+            pts += kv.second;
+        }
+        L.conv_total = pts;
+        L.conv_of.resize(count);
+        for (size_t i = 0; i < count; ++i) L.conv_of[i] = task_of[L.inst[i].bases];
+        L.o_conv = take(pts * 64);
+        L.o_convdesc = take(L.conv.size() * sizeof(MsmConvTask));
+    }
     L.bytes = off + 256;
     return L;
 }
@@ -489,13 +644,23 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     xyzz_t<F>* win = (xyzz_t<F>*)(base + L.o_win);
     hipStream_t st = ctx->stream;
 
+    if (L.conv_total) {
+        TimedLaunch tl(ctx, "msm_convert");
+        const uint64_t blocks = (L.conv_total + 255) / 256, cap = (uint64_t)ctx->num_cus * 32;
+        hipLaunchKernelGGL(k_msm_convert_g1, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st,
+                           (const MsmConvTask*)(base + L.o_convdesc), (uint32_t)L.conv.size(), L.conv_total);
+    }
     msm_launch_scalar_stages(ctx, b, L.tot, L.max_nb, sb);
     {
         TimedLaunch tl(ctx, "msm_accumulate");
         uint64_t blocks = (L.tot[SP_CHUNK] + 255) / 256;
         uint64_t cap = (uint64_t)ctx->num_cus * 16;   // persistent grid: 16 waves per CU
-        hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                           (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
+        if constexpr (sizeof(F) == 32)
+            hipLaunchKernelGGL(k_msm_accumulate_g1, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
+                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
+        else
+            hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
+                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
     }
     {
         TimedLaunch tl(ctx, "msm_fixup");
@@ -521,6 +686,7 @@ template <class F>
 static std::string shape_key(const MsmBatchLayout& L) {
     std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
     k += L.regime == MSM_REGIME_BATCH ? "b:" : "s:";
+    k += std::to_string(L.conv.size()) + "/" + std::to_string(L.conv_total) + ":";
     for (const MsmInst& d : L.inst) {
         char buf[64];
         snprintf(buf, sizeof buf, "%u/%u/%u/%u/%u/%u;", d.n, d.c, d.K, d.g, d.T, d.Lg);
@@ -552,18 +718,28 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
               ctx->lds_per_block);
     int rc = ensure_scratch(ctx, L.bytes);
     if (rc) return rc;
-    const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 + 1024;
+    const size_t conv_bytes = L.conv.size() * sizeof(MsmConvTask);
+    const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 +
+                              conv_bytes + 1024;
     char* pin;
     int slot;
     rc = stage_acquire(ctx, host_bytes, &pin, &slot);
     if (rc) return rc;
     char* base = (char*)ctx->scratch;
     hipStream_t st = ctx->stream;
+    // G1: instances read their converted base arrays in the scratch
+    for (MsmConvTask& t : L.conv) t.dst = base + L.o_conv + (size_t)(uintptr_t)t.dst;
+    for (size_t i = 0; i < L.conv_of.size(); ++i) L.inst[i].bases = L.conv[L.conv_of[i]].dst;
     // descriptors and work-space starts
     std::memcpy(pin, L.inst.data(), count * sizeof(MsmInst));
     std::memcpy(pin + count * sizeof(MsmInst), L.begin.data(), L.begin.size() * 8);
     HZK_HIP(ctx, hipMemcpyAsync(base + L.o_inst, pin, count * sizeof(MsmInst), hipMemcpyHostToDevice, st));
     HZK_HIP(ctx, hipMemcpyAsync(base + L.o_begin, pin + count * sizeof(MsmInst), L.begin.size() * 8, hipMemcpyHostToDevice, st));
+    char* pconv = pin + count * sizeof(MsmInst) + L.begin.size() * 8 + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16;
+    if (conv_bytes) {
+        std::memcpy(pconv, L.conv.data(), conv_bytes);
+        HZK_HIP(ctx, hipMemcpyAsync(base + L.o_convdesc, pconv, conv_bytes, hipMemcpyHostToDevice, st));
+    }
 
     if (capture) {
         std::string key = shape_key<F>(L);
