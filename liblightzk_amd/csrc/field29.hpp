@@ -136,6 +136,64 @@ struct fq29 {
         }
         r.l[8] = (uint32_t)acc;
     }
+    // N independent products in one interleaved scan: each multiply-add of
+    // product c depends only on product c's accumulator, so interleaving
+    // hides the multiply-add latency that a single product's chain exposes
+    template <int N>
+    static HZK_DEV void mont_mul_n(fq29* r, const fq29* a, const fq29* b) {
+        uint32_t m[N][9];
+        uint64_t acc[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) acc[c] = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+#pragma unroll
+            for (int j = 0; j < k; ++j)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+#pragma unroll
+            for (int j = 0; j <= k; ++j)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[j] * b[c].l[k - j];
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                m[c][k] = ((uint32_t)acc[c] * M0) & MASK;
+                acc[c] += (uint64_t)m[c][k] * P[0];
+                acc[c] >>= 29;
+            }
+        }
+#pragma unroll
+        for (int k = 9; k < 17; ++k) {
+#pragma unroll
+            for (int j = k - 8; j < 9; ++j)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[j] * b[c].l[k - j];
+#pragma unroll
+            for (int j = k - 8; j < 9; ++j)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                r[c].l[k - 9] = (uint32_t)acc[c] & MASK;
+                acc[c] >>= 29;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c) r[c].l[8] = (uint32_t)acc[c];
+    }
+    static HZK_DEV void mul2(fq29& r0, const fq29& a0, const fq29& b0, fq29& r1, const fq29& a1, const fq29& b1) {
+        const fq29 a[2] = {a0, a1}, b[2] = {b0, b1};
+        fq29 r[2];
+        mont_mul_n<2>(r, a, b);
+        r0 = r[0], r1 = r[1];
+    }
+    static HZK_DEV void mul3(fq29& r0, const fq29& a0, const fq29& b0, fq29& r1, const fq29& a1, const fq29& b1, fq29& r2,
+                             const fq29& a2, const fq29& b2) {
+        const fq29 a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2};
+        fq29 r[3];
+        mont_mul_n<3>(r, a, b);
+        r0 = r[0], r1 = r[1], r2 = r[2];
+    }
     friend HZK_DEV fq29 operator*(const fq29& a, const fq29& b) { fq29 r; mont_mul(r, a, b); return r; }
     HZK_DEV fq29 sqr() const { return *this * *this; }
 
@@ -226,7 +284,8 @@ struct g1acc29 {
     // this += (x, y) for a finite affine point (x, y < p), every special case handled
     HZK_DEV void add_affine(const fq29& x, const fq29& y) {
         if (inf) { set_affine(x, y); return; }
-        const fq29 U2 = x * ZZ, S2 = y * ZZZ;            // 1 x 2 each
+        fq29 U2, S2;
+        fq29::mul2(U2, x, ZZ, S2, y, ZZZ);               // 1 x 2 each
         const fq29 P = fq29::sub(U2, X, fq29::P8);       // < 10
         const fq29 R = fq29::sub(S2, Y, fq29::P4);       // < 6
         if (P.is_zero_mod_p()) {
@@ -234,20 +293,53 @@ struct g1acc29 {
             else set_inf();
             return;
         }
-        const fq29 PP = P * P, RR = R * R;               // 10 x 10, 6 x 6
-        const fq29 PPP = P * PP, Q = X * PP;             // 10 x 2, 8 x 2
-        const fq29 ZZ3 = ZZ * PP;                        // 2 x 2
+        fq29 PP, RR;
+        fq29::mul2(PP, P, P, RR, R, R);                  // 10 x 10, 6 x 6
+        fq29 PPP, Q, ZZ3;
+        fq29::mul3(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);   // 10 x 2, 8 x 2, 2 x 2
         X = fq29::sub(RR, PPP + Q + Q, fq29::P6);        // RR - (PPP + 2Q) + 6p < 8
         const fq29 D = fq29::sub(Q, X, fq29::P8);        // < 10
-        const fq29 t = R * D, YP = Y * PPP;              // 6 x 10, 4 x 2
-        ZZZ = ZZZ * PPP;                                 // 2 x 2
+        fq29 t, YP, ZZZ3;
+        fq29::mul3(t, R, D, YP, Y, PPP, ZZZ3, ZZZ, PPP); // 6 x 10, 4 x 2, 2 x 2
         Y = fq29::sub(t, YP, fq29::P2);                  // < 4
-        ZZ = ZZ3;
+        ZZ = ZZ3, ZZZ = ZZZ3;
     }
     // the R = 2^256 XYZZ point (canonical coordinates; infinity = all zero)
     HZK_DEV xyzz_t<fq_t> to_xyzz() const {
         if (inf) return xyzz_t<fq_t>::inf();
         return {X.to_r256(), Y.to_r256(), ZZ.to_r256(), ZZZ.to_r256()};
+    }
+    // The raw state, 36 words (X, Y, ZZ, ZZZ limbs; infinity has ZZ = 0, which
+    // a finite point's ZZ = Z^2 never is): a plain store, so a lane that
+    // finishes a bucket in the middle of its chunk holds the others up for a
+    // few instructions instead of four conversions; raw_to_xyzz converts later,
+    // one lane per point.
+    static constexpr int RAW_WORDS = 36;
+    HZK_DEV void store_raw(uint4* dst) const {
+        uint32_t w[RAW_WORDS];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            w[i] = X.l[i], w[9 + i] = Y.l[i], w[18 + i] = inf ? 0u : ZZ.l[i], w[27 + i] = ZZZ.l[i];
+        }
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    }
+    static HZK_DEV xyzz_t<fq_t> raw_to_xyzz(const uint4* src) {
+        uint32_t w[RAW_WORDS];
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) {
+            const uint4 v = src[q];
+            w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
+        }
+        g1acc29 a;
+        uint32_t zz = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            a.X.l[i] = w[i], a.Y.l[i] = w[9 + i], a.ZZ.l[i] = w[18 + i], a.ZZZ.l[i] = w[27 + i];
+            zz |= w[18 + i];
+        }
+        a.inf = zz == 0;
+        return a.to_xyzz();
     }
 };
 

@@ -217,6 +217,7 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
     for (const T& t : tabs) {
         int rc = dev_alloc(ctx, g, t.n * lv * (t.group == 1 ? 64 : 128), t.dst);
         if (!rc) rc = hzk_bases_levels(ctx, t.group, c, (int)lv, t.src, t.n, *t.dst);
+        if (!rc && t.group == 1) rc = msm_convert_g1(ctx, *t.dst, *t.dst, t.n * lv);
         if (rc) return rc;
     }
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -347,6 +348,12 @@ int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
     hipLaunchKernelGGL(k_gather_bitrev64, dim3(blocks_for(4 * nH, 256)), dim3(256), 0, ctx->stream, (const uint4*)g->H1,
                        (uint4*)((char*)g->C1 + nL * 64), nH, g->log_m);
     HZK_HIP(ctx, hipGetLastError());
+    // the G1 MSM arrays in the accumulation's form
+    rc = dev_alloc(ctx, g, g->nA * 64, &g->A1cv);
+    if (!rc) rc = dev_alloc(ctx, g, g->nC * 64, &g->C1v);
+    if (!rc) rc = msm_convert_g1(ctx, g->A1c, g->A1cv, g->nA);
+    if (!rc) rc = msm_convert_g1(ctx, g->C1, g->C1v, g->nC);
+    if (rc) return rc;
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HZK_OK;
 }
@@ -412,8 +419,8 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
         HZK_HIP(ctx, hipMemcpyAsync(c, (char*)reqs[i].z + (l + 1) * 32, nL * 32, hipMemcpyDeviceToDevice, ctx->stream));
         HZK_HIP(ctx, hipMemcpyAsync(c + nL * 32, reqs[i].a, nH * 32, hipMemcpyDeviceToDevice, ctx->stream));
         const uint32_t wc = g->lv > 1 ? (uint32_t)g->lc : 0, L = g->lv > 1 ? g->lv : 0;
-        j1.push_back({b, g->lv > 1 ? g->A1L : g->A1c, g->nA, r, wc, L});
-        j1.push_back({c, g->lv > 1 ? g->C1L : g->C1, g->nC, r + 96, wc, L});
+        j1.push_back({b, g->lv > 1 ? g->A1L : g->A1cv, g->nA, r, wc, L, 1});
+        j1.push_back({c, g->lv > 1 ? g->C1L : g->C1v, g->nC, r + 96, wc, L, 1});
         j2.push_back({b + g->nA * 32, g->lv > 1 ? g->B2L : g->B2c, g->nB, r + 192, wc, L});
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with

@@ -113,8 +113,9 @@ __global__ void __launch_bounds__(256, 2) k_msm_accumulate(MsmBatchDev b, const 
 // buckets and chunk partials are written back in the R = 2^256 form, so the
 // later stages are shared with G2.
 static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev b, const uint32_t* __restrict__ sorted,
-                                                           const uint32_t* __restrict__ offs, g1_xyzz* __restrict__ bucket,
-                                                           g1_xyzz* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
+                                                           const uint32_t* __restrict__ offs, uint4* __restrict__ bucket,
+                                                           uint4* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
+    constexpr int RW = g1acc29::RAW_WORDS / 4;   // uint4 per raw point
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
     const uint64_t total = cbeg[b.ninst];
     for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
@@ -159,10 +160,9 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev
                 acc.add_affine(x, y);
             }
             if (e + 1 == next || e + 1 == e1) {
-                const g1_xyzz r = acc.to_xyzz();
-                if (before) carry[2 * gid] = r;
-                else if (next > e1) carry[2 * gid + 1] = r;
-                else bucket[kb + k] = r;
+                if (before) acc.store_raw(carry + RW * (2 * gid));
+                else if (next > e1) acc.store_raw(carry + RW * (2 * gid + 1));
+                else acc.store_raw(bucket + RW * (kb + k));
                 if (e + 1 < e1) {   // next non-empty bucket
                     do { ++k; next = of[k + 1]; } while (next <= e + 1);
                     before = false;
@@ -171,6 +171,15 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev
             }
         }
     }
+}
+
+// raw accumulator states (g1acc29::store_raw) -> R = 2^256 XYZZ points, one
+// lane per point; unwritten slots (empty buckets, chunks without a partial)
+// convert to values nothing reads
+static __global__ void __launch_bounds__(256) k_msm_unraw_g1(const uint4* __restrict__ raw, g1_xyzz* __restrict__ out, uint64_t n) {
+    constexpr int RW = g1acc29::RAW_WORDS / 4;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = g1acc29::raw_to_xyzz(raw + RW * i);
 }
 
 // one base array to convert for the G1 accumulation: count affine points
@@ -184,6 +193,29 @@ struct MsmConvTask {
 // x, y -> x 2^5, y 2^5 mod p (the 2^261 Montgomery form of field29.hpp),
 // canonical, 8 x 32-bit limbs; the point at infinity (all zero) becomes
 // x word 7 = 0xffffffff, a value no canonical coordinate has
+HZK_DEV void convert_point_g1(const uint4* __restrict__ sp, uint4* __restrict__ dp) {
+    const uint4 a = sp[0], b2 = sp[1], c = sp[2], e = sp[3];
+    const uint32_t any = a.x | a.y | a.z | a.w | b2.x | b2.y | b2.z | b2.w | c.x | c.y | c.z | c.w | e.x | e.y | e.z | e.w;
+    if (!any) {
+        dp[0] = make_uint4(0, 0, 0, 0);
+        dp[1] = make_uint4(0, 0, 0, 0xffffffffu);
+        dp[2] = make_uint4(0, 0, 0, 0);
+        dp[3] = make_uint4(0, 0, 0, 0);
+        return;
+    }
+    const uint32_t xw[8] = {a.x, a.y, a.z, a.w, b2.x, b2.y, b2.z, b2.w};
+    const uint32_t yw[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
+    uint32_t xo[8], yo[8];
+    fq29 x, y;
+    fq29::mul2(x, fq29::unpack(xw), fq29::konst(fq29::TO), y, fq29::unpack(yw), fq29::konst(fq29::TO));
+    x.canonical().pack(xo);
+    y.canonical().pack(yo);
+    dp[0] = make_uint4(xo[0], xo[1], xo[2], xo[3]);
+    dp[1] = make_uint4(xo[4], xo[5], xo[6], xo[7]);
+    dp[2] = make_uint4(yo[0], yo[1], yo[2], yo[3]);
+    dp[3] = make_uint4(yo[4], yo[5], yo[6], yo[7]);
+}
+
 static __global__ void __launch_bounds__(256) k_msm_convert_g1(const MsmConvTask* __restrict__ tasks, uint32_t ntasks, uint64_t total) {
     for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = ntasks;   // tasks[lo].begin <= gid < tasks[hi].begin
@@ -193,27 +225,14 @@ static __global__ void __launch_bounds__(256) k_msm_convert_g1(const MsmConvTask
         }
         const MsmConvTask& t = tasks[lo];
         const uint64_t i = gid - t.begin;
-        const uint4* sp = (const uint4*)t.src + 4 * i;
-        uint4* dp = (uint4*)t.dst + 4 * i;
-        const uint4 a = sp[0], b2 = sp[1], c = sp[2], e = sp[3];
-        const uint32_t any = a.x | a.y | a.z | a.w | b2.x | b2.y | b2.z | b2.w | c.x | c.y | c.z | c.w | e.x | e.y | e.z | e.w;
-        if (!any) {
-            dp[0] = make_uint4(0, 0, 0, 0);
-            dp[1] = make_uint4(0, 0, 0, 0xffffffffu);
-            dp[2] = make_uint4(0, 0, 0, 0);
-            dp[3] = make_uint4(0, 0, 0, 0);
-            continue;
-        }
-        const uint32_t xw[8] = {a.x, a.y, a.z, a.w, b2.x, b2.y, b2.z, b2.w};
-        const uint32_t yw[8] = {c.x, c.y, c.z, c.w, e.x, e.y, e.z, e.w};
-        uint32_t xo[8], yo[8];
-        (fq29::unpack(xw) * fq29::konst(fq29::TO)).canonical().pack(xo);
-        (fq29::unpack(yw) * fq29::konst(fq29::TO)).canonical().pack(yo);
-        dp[0] = make_uint4(xo[0], xo[1], xo[2], xo[3]);
-        dp[1] = make_uint4(xo[4], xo[5], xo[6], xo[7]);
-        dp[2] = make_uint4(yo[0], yo[1], yo[2], yo[3]);
-        dp[3] = make_uint4(yo[4], yo[5], yo[6], yo[7]);
+        convert_point_g1((const uint4*)t.src + 4 * i, (uint4*)t.dst + 4 * i);
     }
+}
+
+// one array, in place or not
+static __global__ void __launch_bounds__(256) k_msm_convert_g1_array(const uint4* src, uint4* dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        convert_point_g1(src + 4 * i, dst + 4 * i);
 }
 
 // ------------------------------------------------------------------ 6. fix-up
@@ -487,6 +506,7 @@ struct MsmBatchLayout {
     std::vector<uint32_t> conv_of;     // instance -> task
     uint64_t conv_total = 0;
     size_t o_conv = 0, o_convdesc = 0;
+    size_t o_rawbucket = 0, o_rawcarry = 0;   // G1: raw accumulator states of buckets and chunk partials
 };
 
 template <class F>
@@ -602,7 +622,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));       // P_p, RS_p per part
     if (!g2) {
         std::map<const void*, uint64_t> need;   // source array -> points referenced (virtual points of level tables)
-        for (const MsmInst& d : L.inst) need[d.bases] = std::max<uint64_t>(need[d.bases], d.n);
+        for (size_t i = 0; i < count; ++i)
+            if (jobs[i].form == 0) need[L.inst[i].bases] = std::max<uint64_t>(need[L.inst[i].bases], L.inst[i].n);
         std::map<const void*, uint32_t> task_of;
         uint64_t pts = 0;
         for (const auto& kv : need) {
@@ -611,10 +632,13 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
             pts += kv.second;
         }
         L.conv_total = pts;
-        L.conv_of.resize(count);
-        for (size_t i = 0; i < count; ++i) L.conv_of[i] = task_of[L.inst[i].bases];
+        L.conv_of.assign(count, UINT32_MAX);   // UINT32_MAX: bases already converted
+        for (size_t i = 0; i < count; ++i)
+            if (jobs[i].form == 0) L.conv_of[i] = task_of[L.inst[i].bases];
         L.o_conv = take(pts * 64);
         L.o_convdesc = take(L.conv.size() * sizeof(MsmConvTask));
+        L.o_rawbucket = take(L.tot[SP_KEY] * g1acc29::RAW_WORDS * 4);
+        L.o_rawcarry = take(2 * L.tot[SP_CHUNK] * g1acc29::RAW_WORDS * 4);
     }
     L.bytes = off + 256;
     return L;
@@ -655,10 +679,17 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
         TimedLaunch tl(ctx, "msm_accumulate");
         uint64_t blocks = (L.tot[SP_CHUNK] + 255) / 256;
         uint64_t cap = (uint64_t)ctx->num_cus * 16;   // persistent grid: 16 waves per CU
-        if constexpr (sizeof(F) == 32)
+        if constexpr (sizeof(F) == 32) {
+            uint4* rb = (uint4*)(base + L.o_rawbucket);
+            uint4* rc = (uint4*)(base + L.o_rawcarry);
             hipLaunchKernelGGL(k_msm_accumulate_g1, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
-        else
+                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, rb, rc, chunk2key);
+            const uint64_t nb = L.tot[SP_KEY], nc = 2 * L.tot[SP_CHUNK], c2 = (uint64_t)ctx->num_cus * 32;
+            hipLaunchKernelGGL(k_msm_unraw_g1, dim3((unsigned)std::min<uint64_t>((nb + 255) / 256, c2)), dim3(256), 0, st,
+                               (const uint4*)rb, (g1_xyzz*)bucket, nb);
+            hipLaunchKernelGGL(k_msm_unraw_g1, dim3((unsigned)std::min<uint64_t>((nc + 255) / 256, c2)), dim3(256), 0, st,
+                               (const uint4*)rc, (g1_xyzz*)carry, nc);
+        } else
             hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
                                (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
     }
@@ -705,6 +736,8 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HZK_CHECK(ctx, (uint64_t)jobs[i].n * (jobs[i].levels ? jobs[i].levels : 1) < (1ull << 31), HZK_ERR_UNSUPPORTED,
                   "msm: n=%zu x %u levels exceeds 2^31 table points", jobs[i].n, jobs[i].levels);
         HZK_CHECK(ctx, jobs[i].scalars && jobs[i].bases && jobs[i].dest, HZK_ERR_INVALID, "msm: null pointer in job %zu", i);
+        HZK_CHECK(ctx, jobs[i].form == 0 || (jobs[i].form == 1 && sizeof(F) == 32), HZK_ERR_INVALID, "msm: base form %u",
+                  jobs[i].form);
         HZK_CHECK(ctx, jobs[i].win_siz == 0 || (jobs[i].win_siz >= (uint32_t)MSM_MIN_C && jobs[i].win_siz <= (uint32_t)MSM_MAX_C),
                   HZK_ERR_INVALID, "msm: win_siz=%u unsupported (0 or %d..%d)", jobs[i].win_siz, MSM_MIN_C, MSM_MAX_C);
     }
@@ -729,7 +762,8 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     hipStream_t st = ctx->stream;
     // G1: instances read their converted base arrays in the scratch
     for (MsmConvTask& t : L.conv) t.dst = base + L.o_conv + (size_t)(uintptr_t)t.dst;
-    for (size_t i = 0; i < L.conv_of.size(); ++i) L.inst[i].bases = L.conv[L.conv_of[i]].dst;
+    for (size_t i = 0; i < L.conv_of.size(); ++i)
+        if (L.conv_of[i] != UINT32_MAX) L.inst[i].bases = L.conv[L.conv_of[i]].dst;
     // descriptors and work-space starts
     std::memcpy(pin, L.inst.data(), count * sizeof(MsmInst));
     std::memcpy(pin + count * sizeof(MsmInst), L.begin.data(), L.begin.size() * 8);

@@ -41,9 +41,13 @@
 
 namespace hzk {
 
-static constexpr int NTT_TILE_BITS = 11;
-static constexpr int NTT_EPT = 8;                                        // elements per thread
-static constexpr int NTT_THREADS = (1 << NTT_TILE_BITS) / NTT_EPT;       // 256
+// Tiles of 2^TB elements, TB = 9, 10 or 11, 2^(TB-3) threads each: a launch
+// takes the largest tile that still gives every CU two workgroups (a lone
+// 2^18 transform: 512 tiles of 2^9; batches and 2^20 and up: 2^11).
+static constexpr int NTT_TILE_BITS = 11;   // largest tile
+static constexpr int NTT_EPT = 8;          // elements per thread
+template <int TB>
+constexpr int ntt_threads() { return (1 << TB) / NTT_EPT; }
 static constexpr int NTT_MAX_LOG = 26;
 
 // one pass of one transform over its tiles [tile_begin, next tile_begin)
@@ -61,14 +65,16 @@ struct NttPassDesc {
 // words; the low 3 bits are XOR-ed with the next 3 so that threads holding
 // 8 consecutive elements (the lowest radix-8 group) spread over the banks
 HZK_DEV uint32_t lds_slot(uint32_t e) { return e ^ ((e >> 3) & 7); }
+template <int TB>
 HZK_DEV void lds_put(uint4* sh, uint32_t e, const fr_t& v) {
     const uint32_t s = lds_slot(e);
     sh[s] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
-    sh[s + (1u << NTT_TILE_BITS)] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+    sh[s + (1u << TB)] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
 }
+template <int TB>
 HZK_DEV fr_t lds_get(const uint4* sh, uint32_t e) {
     const uint32_t s = lds_slot(e);
-    const uint4 a = sh[s], b = sh[s + (1u << NTT_TILE_BITS)];
+    const uint4 a = sh[s], b = sh[s + (1u << TB)];
     fr_t v;
     v.l[0] = a.x, v.l[1] = a.y, v.l[2] = a.z, v.l[3] = a.w, v.l[4] = b.x, v.l[5] = b.y, v.l[6] = b.z, v.l[7] = b.w;
     return v;
@@ -158,7 +164,7 @@ HZK_DEV void mul_table8(fr_t* v, const NttTile& T, const fr_t* __restrict__ tab,
 // [L, L + 3) that contain the chunk's stage bits; its other tile bits come
 // from the thread index (low bits first, so a wave's lanes read consecutive
 // elements wherever L > 0).
-template <bool DIT>
+template <bool DIT, int TB>
 HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const fr_t* __restrict__ pre,
                       const fr_t* __restrict__ post, int k, int s0, int R, int gbits, uint32_t b, uint4* sh) {
     const NttTile T(k, s0, R, gbits, b);
@@ -190,11 +196,11 @@ HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const f
             if (c > 1) __syncthreads();   // every thread has read the previous exchange
             if (active)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) lds_put(sh, eb_prev | ((uint32_t)i << L_prev), v[i]);
+                for (int i = 0; i < 8; ++i) lds_put<TB>(sh, eb_prev | ((uint32_t)i << L_prev), v[i]);
             __syncthreads();
             if (active)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = lds_get(sh, eb | ((uint32_t)i << L));
+                for (int i = 0; i < 8; ++i) v[i] = lds_get<TB>(sh, eb | ((uint32_t)i << L));
         }
         if (active) {
             if (!DIT)
@@ -212,12 +218,12 @@ HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const f
     }
 }
 
-template <bool DIT>
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw,
-                                                          const fr_t* __restrict__ pre, const fr_t* __restrict__ post, int k,
-                                                          int s0, int R, int gbits) {
-    __shared__ uint4 sh[2 << NTT_TILE_BITS];
-    ntt_tile<DIT>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
+template <bool DIT, int TB>
+__global__ void __launch_bounds__(ntt_threads<TB>()) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw,
+                                                                const fr_t* __restrict__ pre, const fr_t* __restrict__ post,
+                                                                int k, int s0, int R, int gbits) {
+    __shared__ uint4 sh[2 << TB];
+    ntt_tile<DIT, TB>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
 }
 
 // transforms of 2 and 4 points (too small for an 8-element group): one thread each
@@ -263,13 +269,14 @@ HZK_DEV uint32_t find_elem(const NttElemDesc* d, uint32_t nd, uint64_t gid) {
     return lo;
 }
 
-__global__ void __launch_bounds__(NTT_THREADS) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
-    __shared__ uint4 sh[2 << NTT_TILE_BITS];
+template <int TB>
+__global__ void __launch_bounds__(ntt_threads<TB>()) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
+    __shared__ uint4 sh[2 << TB];
     const uint32_t i = find_desc(pd, npd, blockIdx.x);
     const NttPassDesc& d = pd[i];
     const uint32_t b = (uint32_t)(blockIdx.x - d.tile_begin);
-    if (d.dit) ntt_tile<true>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
-    else ntt_tile<false>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    if (d.dit) ntt_tile<true, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    else ntt_tile<false, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
 }
 
 __global__ void __launch_bounds__(256) k_bitrev_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
@@ -442,19 +449,20 @@ static int log2_exact(size_t n) {
     return ((size_t)1 << k) == n ? k : -1;
 }
 
-// Pass plan of a 2^k transform in DIF order, (s0, R, gbits) per pass: the
-// lowest min(k, 11) stages (contiguous rows) form the last pass, the rest
-// split evenly into passes of <= 9 stages (>= 4 columns per tile). A DIT run
-// executes the same passes in reverse order. k >= 3.
-static void plan_passes(int k, std::vector<int3>& out) {
+// Pass plan of a 2^k transform in DIF order over tiles of 2^TB elements,
+// (s0, R, gbits) per pass: the lowest min(k, TB) stages (contiguous rows)
+// form the last pass, the rest split evenly into passes of <= TB - 2 stages
+// (>= 4 columns per tile). A DIT run executes the same passes in reverse
+// order. k >= 3.
+static void plan_passes(int k, int TB, std::vector<int3>& out) {
     out.clear();
-    int last = k < NTT_TILE_BITS ? k : NTT_TILE_BITS;
+    int last = k < TB ? k : TB;
     int rest = k - last;
-    int npass = (rest + 8) / 9;
+    int npass = (rest + TB - 3) / (TB - 2);
     int s0 = 0;
     for (int p = 0; p < npass; ++p) {
         int R = rest / npass + (p < rest % npass ? 1 : 0);
-        int gbits = NTT_TILE_BITS - R;
+        int gbits = TB - R;
         int low = k - s0 - R;
         if (gbits > low) gbits = low;
         out.push_back(make_int3(s0, R, gbits));
@@ -466,21 +474,41 @@ static void plan_passes(int k, std::vector<int3>& out) {
 // The passes of one transform on ctx's stream: DIF (natural -> bit-reversed)
 // or DIT (bit-reversed -> natural), pre applied by the first pass executed,
 // post by the last. k >= 3.
+// tile bits for a launch over `elems` elements: the largest tile that still
+// gives every CU two workgroups, within 9..11
+static int tile_bits_for(hzk_ctx* ctx, uint64_t elems) {
+    int TB = NTT_TILE_BITS;
+    while (TB > 9 && (elems >> TB) < 2 * (uint64_t)ctx->num_cus) --TB;
+    return TB;
+}
+
+template <bool DIT, int TB>
+static void launch_pass(hipStream_t st, unsigned tiles, fr_t* x, const fr_t* tw, const fr_t* pp, const fr_t* qq, int k, int3 pl) {
+    hipLaunchKernelGGL((k_ntt_pass<DIT, TB>), dim3(tiles), dim3(ntt_threads<TB>()), 0, st, x, tw, pp, qq, k, pl.x, pl.y, pl.z);
+}
+
 static void ntt_passes(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, int dit, const fr_t* pre, const fr_t* post) {
+    const int TB = std::min(k, tile_bits_for(ctx, (uint64_t)1 << k));
     std::vector<int3> plan;
-    plan_passes(k, plan);
+    plan_passes(k, std::max(TB, 3), plan);
     if (dit) std::reverse(plan.begin(), plan.end());
     for (size_t p = 0; p < plan.size(); ++p) {
         const int3 pl = plan[p];
         const unsigned tiles = (unsigned)(((size_t)1 << k) >> (pl.y + pl.z));
         const fr_t* pp = p == 0 ? pre : nullptr;
         const fr_t* qq = p + 1 == plan.size() ? post : nullptr;
-        if (dit)
-            hipLaunchKernelGGL(k_ntt_pass<true>, dim3(tiles), dim3(NTT_THREADS), 0, ctx->stream, x, tw, pp, qq, k, pl.x,
-                               pl.y, pl.z);
-        else
-            hipLaunchKernelGGL(k_ntt_pass<false>, dim3(tiles), dim3(NTT_THREADS), 0, ctx->stream, x, tw, pp, qq, k, pl.x,
-                               pl.y, pl.z);
+        // the kernel's tile holds 2^(R + gbits) <= 2^TB elements
+        const int tb = pl.y + pl.z;
+        if (tb <= 9) {
+            if (dit) launch_pass<true, 9>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass<false, 9>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+        } else if (tb == 10) {
+            if (dit) launch_pass<true, 10>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass<false, 10>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+        } else {
+            if (dit) launch_pass<true, 11>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass<false, 11>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+        }
     }
 }
 
@@ -567,6 +595,9 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     std::vector<NttTables*> tabs(count);
     std::vector<std::vector<int3>> plans(count);
     size_t rounds = 0;
+    uint64_t all = 0;
+    for (size_t i = 0; i < count; ++i) all += jobs[i].deg;
+    const int TB = tile_bits_for(ctx, all);
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].x && jobs[i].kind >= 0 && jobs[i].kind <= 6, HZK_ERR_INVALID, "ntt batch: bad job %zu", i);
         int k = log2_exact(jobs[i].deg);
@@ -578,7 +609,7 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         if (!rc && jobs[i].kind >= 5) rc = ensure_rev_tables(ctx, k, tabs[i]);
         if (rc) return rc;
         if (k >= 3) {
-            plan_passes(k, plans[i]);
+            plan_passes(k, TB, plans[i]);
             if (jobs[i].kind == 5) std::reverse(plans[i].begin(), plans[i].end());
         }
         rounds = std::max(rounds, plans[i].size());
@@ -648,14 +679,21 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         }
         for (size_t r = 0; r < rounds; ++r) {
             uint32_t np = (uint32_t)(round_begin[r + 1] - round_begin[r]);
-            hipLaunchKernelGGL(k_ntt_pass_batch, dim3((unsigned)round_tiles[r]), dim3(NTT_THREADS), 0, st,
-                               dpd + round_begin[r], np);
+            if (TB == 9)
+                hipLaunchKernelGGL(k_ntt_pass_batch<9>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<9>()), 0, st,
+                                   dpd + round_begin[r], np);
+            else if (TB == 10)
+                hipLaunchKernelGGL(k_ntt_pass_batch<10>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<10>()), 0, st,
+                                   dpd + round_begin[r], np);
+            else
+                hipLaunchKernelGGL(k_ntt_pass_batch<11>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<11>()), 0, st,
+                                   dpd + round_begin[r], np);
         }
         if (elems)
             hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)eds.size(), elems);
     };
     if (capture) {
-        std::string key = "ntt:";
+        std::string key = "ntt:" + std::to_string(TB) + ":";
         for (size_t i = 0; i < count; ++i) {
             key += std::to_string(ks[i]) + "/" + std::to_string(jobs[i].kind) + ";";
             if (ks[i] > 0 && ks[i] < 3) key += std::to_string((uintptr_t)jobs[i].x) + ";";   // launched with direct arguments
