@@ -43,6 +43,8 @@ struct fq29 {
                                        0x0b6d0302, 0x029b8504, 0x197098d0, 0x00c19139};
     static constexpr uint32_t P6[9] = {0x12edefaa, 0x061a4448, 0x0aafd3da, 0x10fed0e5, 0x012318d0,
                                        0x11238484, 0x03e94786, 0x1628e538, 0x012259d6};
+    static constexpr uint32_t P18[9] = {0x18c9cefe, 0x124eccd9, 0x000f7b8e, 0x12fc72b0, 0x03694a71,
+                                        0x136a8d8c, 0x0bbbd693, 0x027aafa8, 0x03670d84};
     static constexpr uint32_t P8[9] = {0x03e7ea38, 0x082305b6, 0x03951a78, 0x16a91687, 0x0c2ecbc0,
                                        0x16da0605, 0x05370a08, 0x12e131a0, 0x01832273};
 
@@ -225,6 +227,22 @@ struct fq29 {
         return c < 0 ? r : d;
     }
 
+    // a normalized value below 2^261 brought below 2p: subtract q p with
+    // q = floor(l8 / (P8 + 1)) <= floor(value / p); what remains is below
+    // (P8 + 1 + q) 2^232 < 2p
+    HZK_DEV fq29 reduce() const {
+        const uint32_t q = l[8] / (P[8] + 1);
+        fq29 r;
+        int64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int64_t v = (int64_t)l[i] - (int64_t)q * P[i] + c;
+            r.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+
     // value == 0 mod p, for a normalized value below 16p: v = k p exactly
     // when k = v p^-1 mod 2^29 is small and k p equals v
     HZK_DEV bool is_zero_mod_p() const {
@@ -266,38 +284,32 @@ struct g1acc29 {
         X = x, Y = y, ZZ = fq29::one(), ZZZ = fq29::one();
         inf = false;
     }
-    // 2 (x, y), mdbl-2008-s-1: U = 2y, V = U^2, W = U V, S = x V, M = 3 x^2,
-    // X3 = M^2 - 2S, Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W
-    HZK_DEV void set_dbl_affine(const fq29& x, const fq29& y) {
-        const fq29 U = (y + y).norm();                   // < 2
-        const fq29 V = U * U, xx = x * x;                // 2 x 2, 1 x 1
-        const fq29 W = U * V, S = x * V;                 // 2 x 2, 1 x 2
-        const fq29 M = (xx + xx + xx).norm();            // < 6
-        const fq29 MM = M * M;                           // 6 x 6
-        X = fq29::sub(MM, S + S, fq29::P4);              // < 6
-        const fq29 D = fq29::sub(S, X, fq29::P8);        // < 10
-        const fq29 t = M * D, Wy = W * y;                // 6 x 10, 2 x 1
-        Y = fq29::sub(t, Wy, fq29::P2);                  // < 4
-        ZZ = V, ZZZ = W;
-        inf = false;
-    }
-    // this += (x, y) for a finite affine point (x, y < p), every special case handled
+    // this += (x, y) for a finite affine point (x, y < p), every special case
+    // handled. When (x, y) equals this point (P = R = 0) the same code doubles
+    // it: dbl-2008-s-1 on the XYZZ state is madd-2008-s with P = 2Y, R = 3X^2
+    // and no PPP term in X3, so the rare case costs one squaring and a select
+    // instead of a second formula's registers.
     HZK_DEV void add_affine(const fq29& x, const fq29& y) {
         if (inf) { set_affine(x, y); return; }
         fq29 U2, S2;
         fq29::mul2(U2, x, ZZ, S2, y, ZZZ);               // 1 x 2 each
-        const fq29 P = fq29::sub(U2, X, fq29::P8);       // < 10
-        const fq29 R = fq29::sub(S2, Y, fq29::P4);       // < 6
+        fq29 P = fq29::sub(U2, X, fq29::P8);             // < 10
+        fq29 R = fq29::sub(S2, Y, fq29::P4);             // < 6
+        bool dbl = false;
         if (P.is_zero_mod_p()) {
-            if (R.is_zero_mod_p()) set_dbl_affine(x, y);
-            else set_inf();
-            return;
+            if (!R.is_zero_mod_p()) { set_inf(); return; }
+            P = (Y + Y).norm();                          // 2Y < 8
+            const fq29 xx = X * X;                       // 8 x 8
+            R = (xx + xx + xx).norm();                   // 3X^2 < 6
+            dbl = true;
         }
         fq29 PP, RR;
         fq29::mul2(PP, P, P, RR, R, R);                  // 10 x 10, 6 x 6
         fq29 PPP, Q, ZZ3;
         fq29::mul3(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);   // 10 x 2, 8 x 2, 2 x 2
-        X = fq29::sub(RR, PPP + Q + Q, fq29::P6);        // RR - (PPP + 2Q) + 6p < 8
+        fq29 T = Q + Q;
+        if (!dbl) T = T + PPP;
+        X = fq29::sub(RR, T, fq29::P6);                  // RR - (PPP + 2Q) + 6p < 8
         const fq29 D = fq29::sub(Q, X, fq29::P8);        // < 10
         fq29 t, YP, ZZZ3;
         fq29::mul3(t, R, D, YP, Y, PPP, ZZZ3, ZZZ, PPP); // 6 x 10, 4 x 2, 2 x 2
@@ -340,6 +352,107 @@ struct g1acc29 {
         }
         a.inf = zz == 0;
         return a.to_xyzz();
+    }
+};
+
+// Fq2 = Fq[u]/(u^2 + 1) over fq29 components
+struct fq2_29 {
+    fq29 c0, c1;
+    static HZK_DEV fq2_29 one() { return {fq29::one(), fq29::zero()}; }
+    friend HZK_DEV fq2_29 operator+(const fq2_29& a, const fq2_29& b) { return {a.c0 + b.c0, a.c1 + b.c1}; }
+    static HZK_DEV fq2_29 sub(const fq2_29& a, const fq2_29& b, const uint32_t* K) {
+        return {fq29::sub(a.c0, b.c0, K), fq29::sub(a.c1, b.c1, K)};
+    }
+    HZK_DEV fq2_29 reduce() const { return {c0.reduce(), c1.reduce()}; }
+    HZK_DEV fq2_29 norm() const { return {c0.norm(), c1.norm()}; }
+    HZK_DEV bool is_zero_mod_p() const { return c0.is_zero_mod_p() && c1.is_zero_mod_p(); }
+    // Karatsuba, 3 interleaved products: c0 = a0 b0 - a1 b1 (< 4p), c1 =
+    // (a0 + a1)(b0 + b1) - a0 b0 - a1 b1 (< 6p); inputs normalized, component
+    // bounds with (A0 + A1)(B0 + B1) < 168
+    friend HZK_DEV fq2_29 operator*(const fq2_29& a, const fq2_29& b) {
+        const fq29 v0 = a.c0 * b.c0, v1 = a.c1 * b.c1, v2 = (a.c0 + a.c1) * (b.c0 + b.c1);
+        return {fq29::sub(v0, v1, fq29::P2), fq29::sub(v2, v0 + v1, fq29::P4)};
+    }
+    // (a0 + a1)(a0 - a1) and 2 a0 a1, 2 interleaved products; components below 2p:
+    // c0 < 2p, c1 < 4p
+    HZK_DEV fq2_29 sqr() const {
+        const fq29 t = (c0 + c1) * fq29::sub(c0, c1, fq29::P2), ab = c0 * c1;
+        return {t, (ab + ab).norm()};
+    }
+};
+
+// G2 bucket accumulator over fq2_29 (madd-2008-s), infinity as a flag.
+// Invariants (component bounds in units of p): X, Y < 2 (reduced after every
+// subtraction), ZZ, ZZZ < 6 (products); affine inputs below p. Every
+// Karatsuba product then sees (A0 + A1)(B0 + B1) <= 100.
+struct g2acc29 {
+    fq2_29 X, Y, ZZ, ZZZ;
+    bool inf;
+
+    HZK_DEV void set_inf() { inf = true; }
+    HZK_DEV void set_affine(const fq2_29& x, const fq2_29& y) {
+        X = x, Y = y, ZZ = fq2_29::one(), ZZZ = fq2_29::one();
+        inf = false;
+    }
+    // this += (x, y), every special case handled; a doubling (x, y equal to
+    // this point) runs the same code with P = 2Y, R = 3X^2 and no PPP term in
+    // X3 (dbl-2008-s-1); ordered for short live ranges (the lane state is 72
+    // registers)
+    HZK_DEV void add_affine(const fq2_29& x, const fq2_29& y) {
+        if (inf) { set_affine(x, y); return; }
+        fq2_29 P = fq2_29::sub(x * ZZ, X, fq29::P2).reduce();                // U2 = x ZZ: (2)(10); P < 2
+        fq2_29 R = fq2_29::sub(y * ZZZ, Y, fq29::P2).reduce();               // S2 = y ZZZ; R < 2
+        bool dbl = false;
+        if (P.is_zero_mod_p()) {
+            if (!R.is_zero_mod_p()) { set_inf(); return; }
+            P = (Y + Y).norm().reduce();                                     // 2Y < 2
+            const fq2_29 xx = X.sqr();
+            R = (xx + xx + xx).norm().reduce();                              // 3X^2 < 2
+            dbl = true;
+        }
+        const fq2_29 PP = P.sqr();                                           // < 4
+        const fq2_29 PPP = P * PP;                                           // (4)(6), < 6
+        ZZ = ZZ * PP;                                                        // (10)(6)
+        const fq2_29 Q = X * PP;                                             // (4)(6), < 6
+        fq2_29 T = Q + Q;
+        if (!dbl) T = T + PPP;
+        X = fq2_29::sub(R.sqr(), T, fq29::P18).reduce();                     // RR - (PPP + 2Q) + 18p, then < 2
+        const fq2_29 t = R * fq2_29::sub(Q, X, fq29::P2).reduce();           // R (Q - X3): (4)(4)
+        Y = fq2_29::sub(t, Y * PPP, fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
+        ZZZ = ZZZ * PPP;                                                     // (10)(10)
+    }
+    // raw state: 72 words, infinity has ZZ = 0
+    static constexpr int RAW_WORDS = 72;
+    HZK_DEV void store_raw(uint4* dst) const {
+        const fq29* c[8] = {&X.c0, &X.c1, &Y.c0, &Y.c1, &ZZ.c0, &ZZ.c1, &ZZZ.c0, &ZZZ.c1};
+        uint32_t w[RAW_WORDS];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) w[9 * k + i] = (inf && (k == 4 || k == 5)) ? 0u : c[k]->l[i];
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    }
+    static HZK_DEV xyzz_t<fq2_t> raw_to_xyzz(const uint4* src) {
+        uint32_t w[RAW_WORDS];
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) {
+            const uint4 v = src[q];
+            w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
+        }
+        uint32_t zz = 0;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) zz |= w[36 + i];
+        if (zz == 0) return xyzz_t<fq2_t>::inf();
+        fq_t o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            fq29 v;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) v.l[i] = w[9 * k + i];
+            o[k] = v.to_r256();
+        }
+        return {{o[0], o[1]}, {o[2], o[3]}, {o[4], o[5]}, {o[6], o[7]}};
     }
 };
 

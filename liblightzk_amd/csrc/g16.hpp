@@ -39,12 +39,12 @@ struct hzk_g16 {
     uint32_t *idxA = nullptr, *idxB = nullptr;
     size_t nA = 0, nB = 0, nC = 0, vec_b = 0, vec_c = 0;
     void *A1c = nullptr, *B2c = nullptr, *C1 = nullptr;
-    // A1c and C1 in the form the G1 bucket accumulation reads (msm_convert_g1),
-    // converted once here instead of in every MSM call
-    void *A1cv = nullptr, *C1v = nullptr;
+    // A1c, C1 and B2c in the form the bucket accumulations read
+    // (msm_convert_g1 / _g2), converted once here instead of in every MSM call
+    void *A1cv = nullptr, *C1v = nullptr, *B2cv = nullptr;
     // fixed-base level tables of the key arrays for window size lc with lv
     // levels (lv <= 1: none); each MSM bucket set then serves lv windows
-    // (A1L and C1L are held in the converted G1 form)
+    // (held in the converted form)
     void *A1L = nullptr, *C1L = nullptr, *B2L = nullptr;
     int lc = 0;
     uint32_t lv = 1;

@@ -217,7 +217,7 @@ int build_levels(hzk_ctx* ctx, hzk_g16* g, uint32_t levels) {
     for (const T& t : tabs) {
         int rc = dev_alloc(ctx, g, t.n * lv * (t.group == 1 ? 64 : 128), t.dst);
         if (!rc) rc = hzk_bases_levels(ctx, t.group, c, (int)lv, t.src, t.n, *t.dst);
-        if (!rc && t.group == 1) rc = msm_convert_g1(ctx, *t.dst, *t.dst, t.n * lv);
+        if (!rc) rc = t.group == 1 ? msm_convert_g1(ctx, *t.dst, *t.dst, t.n * lv) : msm_convert_g2(ctx, *t.dst, *t.dst, t.n * lv);
         if (rc) return rc;
     }
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -351,8 +351,10 @@ int hzk::g16_sparse_key(hzk_ctx* ctx, hzk_g16* g) {
     // the G1 MSM arrays in the accumulation's form
     rc = dev_alloc(ctx, g, g->nA * 64, &g->A1cv);
     if (!rc) rc = dev_alloc(ctx, g, g->nC * 64, &g->C1v);
+    if (!rc) rc = dev_alloc(ctx, g, g->nB * 128, &g->B2cv);
     if (!rc) rc = msm_convert_g1(ctx, g->A1c, g->A1cv, g->nA);
     if (!rc) rc = msm_convert_g1(ctx, g->C1, g->C1v, g->nC);
+    if (!rc) rc = msm_convert_g2(ctx, g->B2c, g->B2cv, g->nB);
     if (rc) return rc;
     HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return HZK_OK;
@@ -421,7 +423,7 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
         const uint32_t wc = g->lv > 1 ? (uint32_t)g->lc : 0, L = g->lv > 1 ? g->lv : 0;
         j1.push_back({b, g->lv > 1 ? g->A1L : g->A1cv, g->nA, r, wc, L, 1});
         j1.push_back({c, g->lv > 1 ? g->C1L : g->C1v, g->nC, r + 96, wc, L, 1});
-        j2.push_back({b + g->nA * 32, g->lv > 1 ? g->B2L : g->B2c, g->nB, r + 192, wc, L});
+        j2.push_back({b + g->nA * 32, g->lv > 1 ? g->B2L : g->B2cv, g->nB, r + 192, wc, L, 1});
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with
     // the G1 batch: the latency-bound tails of one (bucket reduction, window

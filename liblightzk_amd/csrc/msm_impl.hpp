@@ -173,6 +173,73 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev
     }
 }
 
+// G2: the same walk over Fq2 = Fq[u]/(u^2 + 1) in 29-bit-limb components
+// (g2acc29), bases converted by k_msm_convert (128 bytes: x.c0 | x.c1 |
+// y.c0 | y.c1)
+static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g2(MsmBatchDev b, const uint32_t* __restrict__ sorted,
+                                                           const uint32_t* __restrict__ offs, uint4* __restrict__ bucket,
+                                                           uint4* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
+    constexpr int RW = g2acc29::RAW_WORDS / 4;
+    const uint64_t* cbeg = space_g(b, SP_CHUNK);
+    const uint64_t total = cbeg[b.ninst];
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
+        const MsmInst& d = b.inst[ii];
+        const uint32_t E = b.stats[4 * ii];
+        const uint32_t e0 = (uint32_t)(gid - cbeg[ii]) * d.K;
+        if (e0 >= E) continue;
+        const uint32_t e1 = min(e0 + d.K, E);
+        const uint64_t kb = space_g(b, SP_KEY)[ii];
+        const uint32_t* of = offs + kb;
+        uint32_t k = find_key(of, d.nk, e0);
+        chunk2key[gid] = k;
+        uint32_t next = of[k + 1];
+        bool before = of[k] < e0;
+        const uint4* srt4 = (const uint4*)(sorted + space_g(b, SP_ENTRY)[ii]);
+        const uint4* bases = (const uint4*)d.bases;
+        g2acc29 acc;
+        acc.set_inf();
+        uint4 q = srt4[e0 >> 2];
+        uint32_t idx = q.x;
+        for (uint32_t e = e0; e < e1; ++e) {
+            const bool neg = idx >> 31;
+            const uint4* bp = bases + 8 * (size_t)(idx & 0x7fffffffu);
+            const uint4 p0 = bp[0], p1 = bp[1], p2 = bp[2], p3 = bp[3], p4 = bp[4], p5 = bp[5], p6 = bp[6], p7 = bp[7];
+            if (e + 1 < e1) {
+                const uint32_t t = (e + 1) & 3;
+                if (t == 0) q = srt4[(e + 1) >> 2];
+                idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
+            }
+            if (p3.w != 0xffffffffu) {   // the converter's mark for the point at infinity
+                const uint32_t w0[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+                const uint32_t w1[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+                const uint32_t w2[8] = {p4.x, p4.y, p4.z, p4.w, p5.x, p5.y, p5.z, p5.w};
+                const uint32_t w3[8] = {p6.x, p6.y, p6.z, p6.w, p7.x, p7.y, p7.z, p7.w};
+                const fq2_29 x{fq29::unpack(w0), fq29::unpack(w1)};
+                fq2_29 y{fq29::unpack(w2), fq29::unpack(w3)};
+                if (neg) y = {fq29::neg(y.c0, fq29::P1), fq29::neg(y.c1, fq29::P1)};
+                acc.add_affine(x, y);
+            }
+            if (e + 1 == next || e + 1 == e1) {
+                if (before) acc.store_raw(carry + RW * (2 * gid));
+                else if (next > e1) acc.store_raw(carry + RW * (2 * gid + 1));
+                else acc.store_raw(bucket + RW * (kb + k));
+                if (e + 1 < e1) {
+                    do { ++k; next = of[k + 1]; } while (next <= e + 1);
+                    before = false;
+                    acc.set_inf();
+                }
+            }
+        }
+    }
+}
+
+static __global__ void __launch_bounds__(256) k_msm_unraw_g2(const uint4* __restrict__ raw, g2_xyzz* __restrict__ out, uint64_t n) {
+    constexpr int RW = g2acc29::RAW_WORDS / 4;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = g2acc29::raw_to_xyzz(raw + RW * i);
+}
+
 // raw accumulator states (g1acc29::store_raw) -> R = 2^256 XYZZ points, one
 // lane per point; unwritten slots (empty buckets, chunks without a partial)
 // convert to values nothing reads
@@ -216,6 +283,51 @@ HZK_DEV void convert_point_g1(const uint4* __restrict__ sp, uint4* __restrict__ 
     dp[3] = make_uint4(yo[4], yo[5], yo[6], yo[7]);
 }
 
+// G2: x.c0 | x.c1 | y.c0 | y.c1 -> the same conversion per Fq component;
+// infinity (all zero) marked by x.c1 word 7 = 0xffffffff
+HZK_DEV void convert_point_g2(const uint4* __restrict__ sp, uint4* __restrict__ dp) {
+    uint4 v[8];
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[i] = sp[i];
+        any |= v[i].x | v[i].y | v[i].z | v[i].w;
+    }
+    if (!any) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dp[i] = make_uint4(0, 0, 0, i == 3 ? 0xffffffffu : 0);
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c += 2) {
+        const uint32_t a[8] = {v[2 * c].x, v[2 * c].y, v[2 * c].z, v[2 * c].w, v[2 * c + 1].x, v[2 * c + 1].y, v[2 * c + 1].z, v[2 * c + 1].w};
+        const uint32_t b[8] = {v[2 * c + 2].x, v[2 * c + 2].y, v[2 * c + 2].z, v[2 * c + 2].w,
+                               v[2 * c + 3].x, v[2 * c + 3].y, v[2 * c + 3].z, v[2 * c + 3].w};
+        fq29 x, y;
+        fq29::mul2(x, fq29::unpack(a), fq29::konst(fq29::TO), y, fq29::unpack(b), fq29::konst(fq29::TO));
+        uint32_t xo[8], yo[8];
+        x.canonical().pack(xo);
+        y.canonical().pack(yo);
+        dp[2 * c] = make_uint4(xo[0], xo[1], xo[2], xo[3]);
+        dp[2 * c + 1] = make_uint4(xo[4], xo[5], xo[6], xo[7]);
+        dp[2 * c + 2] = make_uint4(yo[0], yo[1], yo[2], yo[3]);
+        dp[2 * c + 3] = make_uint4(yo[4], yo[5], yo[6], yo[7]);
+    }
+}
+
+static __global__ void __launch_bounds__(256) k_msm_convert_g2(const MsmConvTask* __restrict__ tasks, uint32_t ntasks, uint64_t total) {
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ntasks;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tasks[mid].begin <= gid) lo = mid; else hi = mid;
+        }
+        const MsmConvTask& t = tasks[lo];
+        const uint64_t i = gid - t.begin;
+        convert_point_g2((const uint4*)t.src + 8 * i, (uint4*)t.dst + 8 * i);
+    }
+}
+
 static __global__ void __launch_bounds__(256) k_msm_convert_g1(const MsmConvTask* __restrict__ tasks, uint32_t ntasks, uint64_t total) {
     for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = ntasks;   // tasks[lo].begin <= gid < tasks[hi].begin
@@ -233,6 +345,10 @@ static __global__ void __launch_bounds__(256) k_msm_convert_g1(const MsmConvTask
 static __global__ void __launch_bounds__(256) k_msm_convert_g1_array(const uint4* src, uint4* dst, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         convert_point_g1(src + 4 * i, dst + 4 * i);
+}
+static __global__ void __launch_bounds__(256) k_msm_convert_g2_array(const uint4* src, uint4* dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        convert_point_g2(src + 8 * i, dst + 8 * i);
 }
 
 // ------------------------------------------------------------------ 6. fix-up
@@ -620,7 +736,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
     L.o_segsum = take(2 * L.tot[SP_BRED] * sizeof(xyzz_t<F>));   // A_r, R_r per run
     L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));       // P_p, RS_p per part
-    if (!g2) {
+    {
         std::map<const void*, uint64_t> need;   // source array -> points referenced (virtual points of level tables)
         for (size_t i = 0; i < count; ++i)
             if (jobs[i].form == 0) need[L.inst[i].bases] = std::max<uint64_t>(need[L.inst[i].bases], L.inst[i].n);
@@ -628,17 +744,18 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         uint64_t pts = 0;
         for (const auto& kv : need) {
             task_of[kv.first] = (uint32_t)L.conv.size();
-            L.conv.push_back({kv.first, (void*)(uintptr_t)(pts * 64), pts});
+            L.conv.push_back({kv.first, (void*)(uintptr_t)(pts * (g2 ? 128 : 64)), pts});
             pts += kv.second;
         }
         L.conv_total = pts;
         L.conv_of.assign(count, UINT32_MAX);   // UINT32_MAX: bases already converted
         for (size_t i = 0; i < count; ++i)
             if (jobs[i].form == 0) L.conv_of[i] = task_of[L.inst[i].bases];
-        L.o_conv = take(pts * 64);
+        const size_t raw = (g2 ? g2acc29::RAW_WORDS : g1acc29::RAW_WORDS) * 4;
+        L.o_conv = take(pts * (g2 ? 128 : 64));
         L.o_convdesc = take(L.conv.size() * sizeof(MsmConvTask));
-        L.o_rawbucket = take(L.tot[SP_KEY] * g1acc29::RAW_WORDS * 4);
-        L.o_rawcarry = take(2 * L.tot[SP_CHUNK] * g1acc29::RAW_WORDS * 4);
+        L.o_rawbucket = take(L.tot[SP_KEY] * raw);
+        L.o_rawcarry = take(2 * L.tot[SP_CHUNK] * raw);
     }
     L.bytes = off + 256;
     return L;
@@ -671,8 +788,8 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     if (L.conv_total) {
         TimedLaunch tl(ctx, "msm_convert");
         const uint64_t blocks = (L.conv_total + 255) / 256, cap = (uint64_t)ctx->num_cus * 32;
-        hipLaunchKernelGGL(k_msm_convert_g1, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st,
-                           (const MsmConvTask*)(base + L.o_convdesc), (uint32_t)L.conv.size(), L.conv_total);
+        hipLaunchKernelGGL((sizeof(F) == 32 ? k_msm_convert_g1 : k_msm_convert_g2), dim3((unsigned)(blocks < cap ? blocks : cap)),
+                           dim3(256), 0, st, (const MsmConvTask*)(base + L.o_convdesc), (uint32_t)L.conv.size(), L.conv_total);
     }
     msm_launch_scalar_stages(ctx, b, L.tot, L.max_nb, sb);
     {
@@ -689,9 +806,17 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
                                (const uint4*)rb, (g1_xyzz*)bucket, nb);
             hipLaunchKernelGGL(k_msm_unraw_g1, dim3((unsigned)std::min<uint64_t>((nc + 255) / 256, c2)), dim3(256), 0, st,
                                (const uint4*)rc, (g1_xyzz*)carry, nc);
-        } else
-            hipLaunchKernelGGL(k_msm_accumulate<F>, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, bucket, carry, chunk2key);
+        } else {
+            uint4* rb = (uint4*)(base + L.o_rawbucket);
+            uint4* rc = (uint4*)(base + L.o_rawcarry);
+            hipLaunchKernelGGL(k_msm_accumulate_g2, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
+                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, rb, rc, chunk2key);
+            const uint64_t nb = L.tot[SP_KEY], nc = 2 * L.tot[SP_CHUNK], c2 = (uint64_t)ctx->num_cus * 32;
+            hipLaunchKernelGGL(k_msm_unraw_g2, dim3((unsigned)std::min<uint64_t>((nb + 255) / 256, c2)), dim3(256), 0, st,
+                               (const uint4*)rb, (g2_xyzz*)bucket, nb);
+            hipLaunchKernelGGL(k_msm_unraw_g2, dim3((unsigned)std::min<uint64_t>((nc + 255) / 256, c2)), dim3(256), 0, st,
+                               (const uint4*)rc, (g2_xyzz*)carry, nc);
+        }
     }
     {
         TimedLaunch tl(ctx, "msm_fixup");
@@ -736,8 +861,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HZK_CHECK(ctx, (uint64_t)jobs[i].n * (jobs[i].levels ? jobs[i].levels : 1) < (1ull << 31), HZK_ERR_UNSUPPORTED,
                   "msm: n=%zu x %u levels exceeds 2^31 table points", jobs[i].n, jobs[i].levels);
         HZK_CHECK(ctx, jobs[i].scalars && jobs[i].bases && jobs[i].dest, HZK_ERR_INVALID, "msm: null pointer in job %zu", i);
-        HZK_CHECK(ctx, jobs[i].form == 0 || (jobs[i].form == 1 && sizeof(F) == 32), HZK_ERR_INVALID, "msm: base form %u",
-                  jobs[i].form);
+        HZK_CHECK(ctx, jobs[i].form <= 1, HZK_ERR_INVALID, "msm: base form %u", jobs[i].form);
         HZK_CHECK(ctx, jobs[i].win_siz == 0 || (jobs[i].win_siz >= (uint32_t)MSM_MIN_C && jobs[i].win_siz <= (uint32_t)MSM_MAX_C),
                   HZK_ERR_INVALID, "msm: win_siz=%u unsupported (0 or %d..%d)", jobs[i].win_siz, MSM_MIN_C, MSM_MAX_C);
     }
