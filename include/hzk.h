@@ -149,6 +149,20 @@ int hzk_spmv3(hzk_ctx* ctx, const hzk_csr* A, const hzk_csr* B, const hzk_csr* C
  * include/operator/poly.cuh:75-86) */
 int hzk_pointwise_abc(hzk_ctx* ctx, size_t deg, void* A, const void* B, const void* C);
 
+/* G1 MSM that allocates nothing (no counterpart in the reference): for callers
+ * that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same result and layouts as
+ * hzk_msm_g1 with base_layout HZK_BASES_AFFINE and win_siz 0 (dest: device
+ * pointer to 96 bytes). Its scratch is a static device array of
+ * HZK_NOALLOC_SCRATCH_MB MiB per GPU (mapped when a context is created, held
+ * until the process ends); an MSM whose plan needs more (about n > 2^15
+ * points) fails with HZK_ERR_UNSUPPORTED. All calls on one GPU share that
+ * array: calls that overlap on one GPU, from any contexts, race. Host
+ * bookkeeping uses the C++ heap and pageable copies; nothing is allocated on
+ * the device or pinned. */
+#define HZK_NOALLOC_SCRATCH_MB 128
+int hzk_msm_g1_noalloc(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, void* dest);
+
 /* Batched transforms of uneven sizes (no reference counterpart). kind: 0 NTT,
  * 1 inverse NTT (scaled by 1/deg), 2 coset NTT, 3 coset inverse NTT, all with
  * the library's tables; in place. With HZK_BATCH_GRAPH the launches are

@@ -250,6 +250,10 @@ class Context:
     def msm_g1(self, scalars, bases, n, dest, win_siz=0, layout=0):
         self._check(load().hzk_msm_g1(self.handle, int(scalars), int(bases), n, layout, win_siz, int(dest)))
 
+    def msm_g1_noalloc(self, scalars, bases, n, dest):
+        """hzk_msm_g1_noalloc: the G1 MSM on the library's static device scratch (no allocation)."""
+        self._check(load().hzk_msm_g1_noalloc(self.handle, int(scalars), int(bases), n, int(dest)))
+
     def bases_levels(self, group, c, bases, n, out, levels=None):
         """Level table out[j*n + i] = 2^(c j) bases[i] (affine), j < levels (default ceil(255 / c))."""
         levels = levels or (255 + c - 1) // c

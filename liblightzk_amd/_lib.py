@@ -50,6 +50,7 @@ def load():
         "hzk_field_op": (i32, [vp, i32, i32, vp, vp, vp, sz]),
         "hzk_msm_g1": (i32, [vp, vp, vp, sz, u32, sz, vp]),
         "hzk_msm_g2": (i32, [vp, vp, vp, sz, u32, sz, vp]),
+        "hzk_msm_g1_noalloc": (i32, [vp, vp, vp, sz, vp]),
         "hzk_ntt": (i32, [vp, vp, sz, vp, i32, vp]),
         "hzk_coset_ntt": (i32, [vp, vp, sz, i32]),
         "hzk_ntt_setup": (i32, [sz, vp]),

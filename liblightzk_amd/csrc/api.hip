@@ -185,6 +185,7 @@ int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
     hzk_ctx* ctx = new hzk_ctx();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
+    if (msm_noalloc_init(device) != HZK_OK) { delete ctx; return HZK_ERR_HIP; }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         ctx->lds_per_block = prop.sharedMemPerBlock;

@@ -103,6 +103,8 @@ struct Arena {
 };
 // Make sure ctx->scratch holds at least `bytes`; returns 0 or an error code.
 int ensure_scratch(hzk_ctx* ctx, size_t bytes);
+// maps the static scratch of hzk_msm_g1_noalloc for a device (msm_g1.hip)
+int msm_noalloc_init(int device);
 int ctx_aux(hzk_ctx* ctx, hzk_ctx** out);
 void timing_merge(hzk_ctx* ctx, hzk_ctx* from);
 // Pinned host staging of at least `bytes`: *buf is writable now; call

@@ -851,8 +851,12 @@ static std::string shape_key(const MsmBatchLayout& L) {
     return k;
 }
 
+// ext_scratch != null: run on that device buffer (ext_bytes long) instead of
+// the context's arena, with pageable host staging; nothing is allocated on
+// the device or pinned on the host (no graph capture in this mode)
 template <class F, class HostPt>
-int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture) {
+int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture, void* ext_scratch = nullptr,
+                  size_t ext_bytes = 0) {
     HZK_CHECK(ctx, count > 0 && jobs, HZK_ERR_INVALID, "msm: empty batch");
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].n > 0 && jobs[i].n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", jobs[i].n);
@@ -873,16 +877,28 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     HZK_CHECK(ctx, (size_t)L.max_nb * 4 <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
               "msm: a %u-bucket window needs %u B of LDS per workgroup, device allows %zu", L.max_nb, L.max_nb * 4,
               ctx->lds_per_block);
-    int rc = ensure_scratch(ctx, L.bytes);
-    if (rc) return rc;
+    int rc = 0;
+    if (ext_scratch) {
+        HZK_CHECK(ctx, L.bytes <= ext_bytes && !capture, HZK_ERR_UNSUPPORTED,
+                  "msm: this MSM needs %zu bytes of scratch, the static buffer holds %zu", L.bytes, ext_bytes);
+    } else {
+        rc = ensure_scratch(ctx, L.bytes);
+        if (rc) return rc;
+    }
     const size_t conv_bytes = L.conv.size() * sizeof(MsmConvTask);
     const size_t host_bytes = count * sizeof(MsmInst) + L.begin.size() * 8 + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>) + count * 16 +
                               conv_bytes + 1024;
     char* pin;
-    int slot;
-    rc = stage_acquire(ctx, host_bytes, &pin, &slot);
-    if (rc) return rc;
-    char* base = (char*)ctx->scratch;
+    int slot = -1;
+    std::vector<char> pageable;
+    if (ext_scratch) {
+        pageable.resize(host_bytes);
+        pin = pageable.data();
+    } else {
+        rc = stage_acquire(ctx, host_bytes, &pin, &slot);
+        if (rc) return rc;
+    }
+    char* base = ext_scratch ? (char*)ext_scratch : (char*)ctx->scratch;
     hipStream_t st = ctx->stream;
     // G1: instances read their converted base arrays in the scratch
     for (MsmConvTask& t : L.conv) t.dst = base + L.o_conv + (size_t)(uintptr_t)t.dst;
@@ -927,8 +943,10 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     uint32_t* hstats = (uint32_t*)(hwin + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));
     HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));
     HZK_HIP(ctx, hipMemcpyAsync(hstats, base + L.o_stats, count * 16, hipMemcpyDeviceToHost, st));
-    rc = stage_fence(ctx, slot);
-    if (rc) return rc;
+    if (slot >= 0) {
+        rc = stage_fence(ctx, slot);
+        if (rc) return rc;
+    }
     HZK_HIP(ctx, hipStreamSynchronize(st));
 
     // window values on the host, all (instance, window) pairs in parallel:
