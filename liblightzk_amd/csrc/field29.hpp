@@ -324,25 +324,17 @@ struct g1acc29 {
     // The raw state, 36 words (X, Y, ZZ, ZZZ limbs; infinity has ZZ = 0, which
     // a finite point's ZZ = Z^2 never is): a plain store, so a lane that
     // finishes a bucket in the middle of its chunk holds the others up for a
-    // few instructions instead of four conversions; raw_to_xyzz converts later,
-    // one lane per point.
+    // few instructions instead of four conversions. The later stages (bucket
+    // fix-up and reduction) read and write this form too; only the window
+    // sums leave it (to_xyzz).
     static constexpr int RAW_WORDS = 36;
-    HZK_DEV void store_raw(uint4* dst) const {
-        uint32_t w[RAW_WORDS];
+    HZK_DEV void to_words(uint32_t* w) const {
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             w[i] = X.l[i], w[9 + i] = Y.l[i], w[18 + i] = inf ? 0u : ZZ.l[i], w[27 + i] = ZZZ.l[i];
         }
-#pragma unroll
-        for (int q = 0; q < RAW_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     }
-    static HZK_DEV xyzz_t<fq_t> raw_to_xyzz(const uint4* src) {
-        uint32_t w[RAW_WORDS];
-#pragma unroll
-        for (int q = 0; q < RAW_WORDS / 4; ++q) {
-            const uint4 v = src[q];
-            w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
-        }
+    static HZK_DEV g1acc29 from_words(const uint32_t* w) {
         g1acc29 a;
         uint32_t zz = 0;
 #pragma unroll
@@ -351,7 +343,66 @@ struct g1acc29 {
             zz |= w[18 + i];
         }
         a.inf = zz == 0;
-        return a.to_xyzz();
+        return a;
+    }
+    HZK_DEV void store_raw(uint4* dst) const {
+        uint32_t w[RAW_WORDS];
+        to_words(w);
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+    }
+    static HZK_DEV g1acc29 load_raw(const uint4* src) {
+        uint32_t w[RAW_WORDS];
+#pragma unroll
+        for (int q = 0; q < RAW_WORDS / 4; ++q) {
+            const uint4 v = src[q];
+            w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
+        }
+        return from_words(w);
+    }
+
+    // this += q for two states under the invariants (add-2008-s); when q
+    // equals this point the same tail doubles it (dbl-2008-s-1 as in
+    // add_affine: U1 -> X, S1 -> Y, the ZZ, ZZZ products -> this point's)
+    HZK_DEV void add(const g1acc29& q) {
+        if (q.inf) return;
+        if (inf) { *this = q; return; }
+        fq29 U1, U2, ZZ12, S1, S2, ZZZ12;
+        fq29::mul3(U1, X, q.ZZ, U2, q.X, ZZ, ZZ12, ZZ, q.ZZ);        // 8 x 2, 8 x 2, 2 x 2
+        fq29::mul3(S1, Y, q.ZZZ, S2, q.Y, ZZZ, ZZZ12, ZZZ, q.ZZZ);   // 4 x 2, 4 x 2, 2 x 2
+        fq29 P = fq29::sub(U2, U1, fq29::P2);                        // < 4
+        fq29 R = fq29::sub(S2, S1, fq29::P2);                        // < 4
+        bool d = false;
+        if (P.is_zero_mod_p()) {
+            if (!R.is_zero_mod_p()) { set_inf(); return; }
+            P = (Y + Y).norm();                                      // 2Y < 8
+            const fq29 xx = X * X;                                   // 8 x 8
+            R = (xx + xx + xx).norm();                               // 3X^2 < 6
+            U1 = X, S1 = Y, ZZ12 = ZZ, ZZZ12 = ZZZ;
+            d = true;
+        }
+        finish(P, R, U1, S1, ZZ12, ZZZ12, d);
+    }
+    HZK_DEV void dbl() {
+        if (inf) return;
+        const fq29 xx = X * X;
+        finish((Y + Y).norm(), (xx + xx + xx).norm(), X, Y, ZZ, ZZZ, true);
+    }
+    // the common tail: P < 8, R < 6, U1 < 8, S1 < 4, ZZ12, ZZZ12 < 2
+    HZK_DEV void finish(const fq29& P, const fq29& R, const fq29& U1, const fq29& S1, const fq29& ZZ12, const fq29& ZZZ12,
+                        bool d) {
+        fq29 PP, RR;
+        fq29::mul2(PP, P, P, RR, R, R);                              // 8 x 8, 6 x 6
+        fq29 PPP, Q, ZZn;
+        fq29::mul3(PPP, P, PP, Q, U1, PP, ZZn, ZZ12, PP);            // 8 x 2, 8 x 2, 2 x 2
+        fq29 T = Q + Q;
+        if (!d) T = T + PPP;
+        X = fq29::sub(RR, T, fq29::P6);                              // < 8
+        const fq29 D = fq29::sub(Q, X, fq29::P8);                    // < 10
+        fq29 t, YP, ZZZn;
+        fq29::mul3(t, R, D, YP, S1, PPP, ZZZn, ZZZ12, PPP);          // 6 x 10, 4 x 2, 2 x 2
+        Y = fq29::sub(t, YP, fq29::P2);                              // < 4
+        ZZ = ZZn, ZZZ = ZZZn;
     }
 };
 
@@ -372,6 +423,19 @@ struct fq2_29 {
     friend HZK_DEV fq2_29 operator*(const fq2_29& a, const fq2_29& b) {
         const fq29 v0 = a.c0 * b.c0, v1 = a.c1 * b.c1, v2 = (a.c0 + a.c1) * (b.c0 + b.c1);
         return {fq29::sub(v0, v1, fq29::P2), fq29::sub(v2, v0 + v1, fq29::P4)};
+    }
+    // the same products in one interleaved scan (three independent chains of
+    // multiply-adds): for kernels at one wave per SIMD, where a single
+    // product's dependent chain would leave the SIMD waiting
+    static HZK_DEV fq2_29 mul_x(const fq2_29& a, const fq2_29& b) {
+        fq29 v0, v1, v2;
+        fq29::mul3(v0, a.c0, b.c0, v1, a.c1, b.c1, v2, a.c0 + a.c1, b.c0 + b.c1);
+        return {fq29::sub(v0, v1, fq29::P2), fq29::sub(v2, v0 + v1, fq29::P4)};
+    }
+    HZK_DEV fq2_29 sqr_x() const {
+        fq29 t, ab;
+        fq29::mul2(t, c0 + c1, fq29::sub(c0, c1, fq29::P2), ab, c0, c1);
+        return {t, (ab + ab).norm()};
     }
     // (a0 + a1)(a0 - a1) and 2 a0 a1, 2 interleaved products; components below 2p:
     // c0 < 2p, c1 < 4p
@@ -421,38 +485,92 @@ struct g2acc29 {
         Y = fq2_29::sub(t, Y * PPP, fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
         ZZZ = ZZZ * PPP;                                                     // (10)(10)
     }
-    // raw state: 72 words, infinity has ZZ = 0
+    // raw state: 72 words, infinity has ZZ = 0 (read back by load_raw)
     static constexpr int RAW_WORDS = 72;
-    HZK_DEV void store_raw(uint4* dst) const {
+    HZK_DEV void to_words(uint32_t* w) const {
         const fq29* c[8] = {&X.c0, &X.c1, &Y.c0, &Y.c1, &ZZ.c0, &ZZ.c1, &ZZZ.c0, &ZZZ.c1};
-        uint32_t w[RAW_WORDS];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
 #pragma unroll
             for (int i = 0; i < 9; ++i) w[9 * k + i] = (inf && (k == 4 || k == 5)) ? 0u : c[k]->l[i];
+    }
+    static HZK_DEV g2acc29 from_words(const uint32_t* w) {
+        g2acc29 a;
+        fq29* c[8] = {&a.X.c0, &a.X.c1, &a.Y.c0, &a.Y.c1, &a.ZZ.c0, &a.ZZ.c1, &a.ZZZ.c0, &a.ZZZ.c1};
+        uint32_t zz = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) c[k]->l[i] = w[9 * k + i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) zz |= w[36 + i];
+        a.inf = zz == 0;
+        return a;
+    }
+    HZK_DEV void store_raw(uint4* dst) const {
+        uint32_t w[RAW_WORDS];
+        to_words(w);
 #pragma unroll
         for (int q = 0; q < RAW_WORDS / 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
     }
-    static HZK_DEV xyzz_t<fq2_t> raw_to_xyzz(const uint4* src) {
+    static HZK_DEV g2acc29 load_raw(const uint4* src) {
         uint32_t w[RAW_WORDS];
 #pragma unroll
         for (int q = 0; q < RAW_WORDS / 4; ++q) {
             const uint4 v = src[q];
             w[4 * q] = v.x, w[4 * q + 1] = v.y, w[4 * q + 2] = v.z, w[4 * q + 3] = v.w;
         }
-        uint32_t zz = 0;
-#pragma unroll
-        for (int i = 0; i < 18; ++i) zz |= w[36 + i];
-        if (zz == 0) return xyzz_t<fq2_t>::inf();
-        fq_t o[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            fq29 v;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) v.l[i] = w[9 * k + i];
-            o[k] = v.to_r256();
+        return from_words(w);
+    }
+    // the R = 2^256 XYZZ point (canonical components; infinity = all zero)
+    HZK_DEV xyzz_t<fq2_t> to_xyzz() const {
+        if (inf) return xyzz_t<fq2_t>::inf();
+        return {{X.c0.to_r256(), X.c1.to_r256()}, {Y.c0.to_r256(), Y.c1.to_r256()},
+                {ZZ.c0.to_r256(), ZZ.c1.to_r256()}, {ZZZ.c0.to_r256(), ZZZ.c1.to_r256()}};
+    }
+
+    // this += q for two states under the invariants (add-2008-s); when q
+    // equals this point the same tail doubles it (dbl-2008-s-1 with U1 -> X,
+    // S1 -> Y and the ZZ, ZZZ products -> this point's). Interleaved
+    // products (mul_x): the callers run at one wave per SIMD.
+    HZK_DEV void add(const g2acc29& q) {
+        if (q.inf) return;
+        if (inf) { *this = q; return; }
+        fq2_29 U1 = fq2_29::mul_x(X, q.ZZ);                                  // (4)(10) -> c0 < 4, c1 < 6
+        fq2_29 P = fq2_29::sub(fq2_29::mul_x(q.X, ZZ), U1, fq29::P6).reduce();   // U2 - U1, < 2
+        fq2_29 S1 = fq2_29::mul_x(Y, q.ZZZ);
+        fq2_29 R = fq2_29::sub(fq2_29::mul_x(q.Y, ZZZ), S1, fq29::P6).reduce();  // S2 - S1, < 2
+        bool d = false;
+        if (P.is_zero_mod_p()) {
+            if (!R.is_zero_mod_p()) { set_inf(); return; }
+            P = (Y + Y).norm().reduce();
+            const fq2_29 xx = X.sqr_x();
+            R = (xx + xx + xx).norm().reduce();
+            U1 = X, S1 = Y;
+            d = true;
         }
-        return {{o[0], o[1]}, {o[2], o[3]}, {o[4], o[5]}, {o[6], o[7]}};
+        const fq2_29 ZZ12 = d ? ZZ : fq2_29::mul_x(ZZ, q.ZZ);               // (10)(10) -> (4, 6)
+        const fq2_29 ZZZ12 = d ? ZZZ : fq2_29::mul_x(ZZZ, q.ZZZ);
+        finish(P, R, U1, S1, ZZ12, ZZZ12, d);
+    }
+    HZK_DEV void dbl() {
+        if (inf) return;
+        const fq2_29 xx = X.sqr_x();
+        finish((Y + Y).norm().reduce(), (xx + xx + xx).norm().reduce(), X, Y, ZZ, ZZZ, true);
+    }
+    // the common tail: P, R < 2; U1, S1, ZZ12, ZZZ12 with c0 < 4, c1 < 6
+    HZK_DEV void finish(const fq2_29& P, const fq2_29& R, const fq2_29& U1, const fq2_29& S1, const fq2_29& ZZ12,
+                        const fq2_29& ZZZ12, bool d) {
+        const fq2_29 PP = P.sqr_x();                                         // (2, 4)
+        const fq2_29 PPP = fq2_29::mul_x(P, PP);                             // (4)(6)
+        ZZ = fq2_29::mul_x(ZZ12, PP);                                        // (10)(6)
+        ZZZ = fq2_29::mul_x(ZZZ12, PPP);                                     // (10)(10)
+        const fq2_29 Q = fq2_29::mul_x(U1, PP);                              // (10)(6)
+        fq2_29 T = Q + Q;
+        if (!d) T = T + PPP;                                                 // c0 < 12, c1 < 18
+        X = fq2_29::sub(R.sqr_x(), T, fq29::P18).reduce();                   // < 2
+        const fq2_29 t = fq2_29::mul_x(R, fq2_29::sub(Q, X, fq29::P2).reduce());   // (4)(4)
+        Y = fq2_29::sub(t, fq2_29::mul_x(S1, PPP), fq29::P6).reduce();       // S1 PPP: (10)(10); Y3 < 2
     }
 };
 

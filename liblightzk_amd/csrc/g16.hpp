@@ -77,8 +77,15 @@ int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n);
 // the MSMs of several requests as two batched MSMs (G1: A and the merged C
 // MSM per request, G2: B per request); results land in reqs[i].res on the host
 int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph);
+// the proof terms that depend only on the key and r, s: r delta1, s delta2,
+// r beta1 (host scalar multiplications, computed while the device proves)
+struct G16Fixed {
+    host::G1x r_delta1, r_beta1;
+    host::G2x s_delta2;
+};
+G16Fixed g16_fixed_terms(const hzk_g16* g, const void* rs);
 // phase-6 assembly on the host; proof = A | B | C (affine, 256 B)
-void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof);
+void g16_assemble(const hzk_g16* g, const char* res, const void* rs, const G16Fixed& fx, void* proof);
 // device bytes one request needs beyond its buffers (MSM scratch, generous)
 size_t g16_request_footprint(const hzk_g16* g);
 // allocate the plain prover's buffers (z, a, b, c) and upload 1/(g^M - 1)

@@ -451,9 +451,11 @@ int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
 
 // Upper bound of one MSM's scratch (digits and sorted entries, histograms,
 // key scans, bucket sums, chunk partials, bucket-reduction and window sums)
-// for n points with xyzz points of pt_bytes, from the built-in plans (the
-// larger of the single-MSM and wide-batch regimes).
+// for n points with xyzz points of pt_bytes (bucket sums, chunk partials and
+// run sums are raw 29-bit accumulator states, 9/8 of that), from the
+// built-in plans (the larger of the single-MSM and wide-batch regimes).
 static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
+    const size_t raw = pt_bytes / 8 * 9;
     size_t worst = 0;
     for (int regime : {MSM_REGIME_SINGLE, MSM_REGIME_BATCH}) {
         const MsmPlan p = msm_plan_builtin(pt_bytes > 128, n, regime, 160 * 1024);
@@ -462,8 +464,8 @@ static size_t msm_scratch_bound(size_t n, size_t pt_bytes) {
         size_t bred = W * ((p.NB + p.g - 1) / p.g);
         const size_t coarse = p.NB >= 2048 && n >= MSM_TP_BATCH_MIN_N ? entries * 8 : 0;   // two-pass staging
         size_t b = entries * 8 + coarse + (size_t)p.T * nk * 4 + 2 * (nk + 1) * 4 +
-                   2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 + (nk + 1) * pt_bytes + chunks * (4 + 2 * pt_bytes) +
-                   2 * (bred + 64 * W) * pt_bytes + 4096;   // run (A, R) and part (P, RS) pairs
+                   2 * ((nk + 1 + MSM_SCAN_TILE) / MSM_SCAN_TILE) * 8 + (nk + 1) * raw + chunks * (4 + 2 * raw) +
+                   2 * bred * raw + 2 * 64 * W * pt_bytes + 4096;   // run (A, R) and part (P, RS) pairs
         worst = std::max(worst, b);
     }
     return worst;
@@ -476,14 +478,22 @@ size_t g16_request_footprint(const hzk_g16* g) {
     return bufs + msm + msm / 4;
 }
 
+G16Fixed g16_fixed_terms(const hzk_g16* g, const void* rs) {
+    Fr r = Fr::load(rs), s = Fr::load((const char*)rs + 32);
+    uint64_t rc[4], sc[4];
+    r.canonical(rc);
+    s.canonical(sc);
+    return {g->delta1.mul(rc), g->beta1.mul(rc), g->delta2.mul(sc)};
+}
+
 // Phase 6 on the host; proof = A (64 B) | B (128 B) | C (64 B), affine.
 // With the merged C MSM (sum L + sum H + sum (r z_i) B1_i):
 //   A = alpha1 + sum A + r delta1, B = beta2 + sum B2 + s delta2,
 //   C = merged + s A + r beta1
 // (= sum L + sum H + s A + r B1 - r s delta1 with B1 = beta1 + sum B1 + s delta1,
-// groth16.cuh:210-215).
-void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof) {
-    Fr r = Fr::load(rs), s = Fr::load((const char*)rs + 32);
+// groth16.cuh:210-215). fx holds r delta1, s delta2 and r beta1.
+void g16_assemble(const hzk_g16* g, const char* res, const void* rs, const G16Fixed& fx, void* proof) {
+    Fr s = Fr::load((const char*)rs + 32);
     auto jac1 = [](const char* p) {   // normalised Jacobian (x, y, 1) or (0, 1, 0)
         Fq z = Fq::load(p + 64);
         if (z.is_zero()) return G1x::inf();
@@ -496,12 +506,11 @@ void g16_assemble(const hzk_g16* g, const char* res, const void* rs, void* proof
     };
     G1x sA = jac1(res), merged = jac1(res + 96);
     G2x sB2 = jac2(res + 192);
-    uint64_t rc[4], sc[4];
-    r.canonical(rc);
+    uint64_t sc[4];
     s.canonical(sc);
-    G1x A = g->alpha1.add(sA).add(g->delta1.mul(rc));
-    G2x B = g->beta2.add(sB2).add(g->delta2.mul(sc));
-    G1x C = merged.add(A.mul(sc)).add(g->beta1.mul(rc));
+    G1x A = g->alpha1.add(sA).add(fx.r_delta1);
+    G2x B = g->beta2.add(sB2).add(fx.s_delta2);
+    G1x C = merged.add(A.mul(sc)).add(fx.r_beta1);
     char* out = (char*)proof;
     A.store_affine(out);
     B.store_affine(out + 64);
@@ -610,10 +619,13 @@ int hzk_g16_prove(hzk_ctx* ctx, hzk_g16* g, const void* z_host, const void* rs_h
     HZK_CHECK(ctx, z_host && rs_host && proof_host, HZK_ERR_INVALID, "g16_prove: null argument");
     HZK_HIP(ctx, hipMemcpyAsync(g->z, z_host, g->num_vars * 32, hipMemcpyHostToDevice, ctx->stream));
     G16Req r{g, g->z, g->a, g->b, g->c, rs_host, {}};
+    G16Fixed fx;
+    std::thread fixed([&] { fx = g16_fixed_terms(g, rs_host); });   // host work beside the device's
     int rc = g16_h_batch(ctx, &r, 1);
     if (!rc) rc = g16_msm_batch(ctx, &r, 1, 0);
+    fixed.join();
     if (rc) return rc;
-    g16_assemble(g, r.res, rs_host, proof_host);
+    g16_assemble(g, r.res, rs_host, fx, proof_host);
     return HZK_OK;
 }
 

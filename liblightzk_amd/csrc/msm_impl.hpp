@@ -50,68 +50,12 @@ HZK_DEV uint32_t find_key(const uint32_t* __restrict__ offs, uint32_t nk, uint32
 // to bucket[k]; the partial of a bucket that began in an earlier chunk goes
 // to carry[2c] (head), that of a bucket continuing into the next chunk to
 // carry[2c+1] (tail). Persistent grid over the whole batch's chunk space.
-// at least two waves per SIMD: the G2 lane state does not fit twice in 512
-// registers otherwise (occupancy 1 left the G2 accumulation latency-bound)
-template <class F>
-__global__ void __launch_bounds__(256, 2) k_msm_accumulate(MsmBatchDev b, const uint32_t* __restrict__ sorted,
-                                                        const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ bucket,
-                                                        xyzz_t<F>* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
-    const uint64_t* cbeg = space_g(b, SP_CHUNK);
-    const uint64_t total = cbeg[b.ninst];
-    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
-        const MsmInst& d = b.inst[ii];
-        const uint32_t E = b.stats[4 * ii];
-        const uint32_t e0 = (uint32_t)(gid - cbeg[ii]) * d.K;
-        if (e0 >= E) continue;
-        const uint32_t e1 = min(e0 + d.K, E);
-        const uint64_t kb = space_g(b, SP_KEY)[ii];
-        const uint32_t* of = offs + kb;
-        uint32_t k = find_key(of, d.nk, e0);
-        chunk2key[gid] = k;
-        uint32_t next = of[k + 1];
-        bool before = of[k] < e0;
-        // sorted indices come in 16-byte groups (instance ranges are 256-B
-        // aligned and K is a multiple of 4): one load per four entries keeps
-        // each lane's 128-byte index line from being fetched again and again
-        const uint4* srt4 = (const uint4*)(sorted + space_g(b, SP_ENTRY)[ii]);
-        const affine_t<F>* bases = (const affine_t<F>*)d.bases;
-        xyzz_t<F> acc = xyzz_t<F>::inf();
-        uint4 q = srt4[e0 >> 2];
-        uint32_t idx = q.x;
-        affine_t<F> P = bases[idx & 0x7fffffffu];
-        for (uint32_t e = e0; e < e1; ++e) {
-            bool neg = idx >> 31;
-            affine_t<F> Q = P;
-            constexpr bool prefetch = sizeof(F) == 32;   // G2: registers are scarcer than latency
-            if (e + 1 < e1) {   // prefetch the next base while this one is added
-                const uint32_t t = (e + 1) & 3;
-                if (t == 0) q = srt4[(e + 1) >> 2];
-                idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
-                if (prefetch) P = bases[idx & 0x7fffffffu];
-            }
-            if (neg) Q.y = Q.y.neg();
-            acc.add_affine(Q);
-            if (!prefetch && e + 1 < e1) P = bases[idx & 0x7fffffffu];
-            if (e + 1 == next || e + 1 == e1) {
-                if (before) carry[2 * gid] = acc;
-                else if (next > e1) carry[2 * gid + 1] = acc;
-                else bucket[kb + k] = acc;
-                if (e + 1 < e1) {   // next non-empty bucket
-                    do { ++k; next = of[k + 1]; } while (next <= e + 1);
-                    before = false;
-                    acc = xyzz_t<F>::inf();
-                }
-            }
-        }
-    }
-}
-
-// G1: the same chunk walk with the additions in 29-bit-limb arithmetic
-// (field29.hpp: one v_mad_u64_u32 per partial product, lazy reduction) over
-// the bases converted to its Montgomery form by k_msm_convert_g1; complete
-// buckets and chunk partials are written back in the R = 2^256 form, so the
-// later stages are shared with G2.
+// G1: the additions in 29-bit-limb arithmetic (field29.hpp: one
+// v_mad_u64_u32 per partial product, lazy reduction) over the bases converted
+// to its Montgomery form by k_msm_convert_g1. Bucket sums and chunk partials
+// are stored as raw accumulator states (g1acc29::store_raw), the form the
+// fix-up and the bucket reduction work in. At least two waves per SIMD (the
+// G2 lane state does not fit twice in 512 registers otherwise).
 static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev b, const uint32_t* __restrict__ sorted,
                                                            const uint32_t* __restrict__ offs, uint4* __restrict__ bucket,
                                                            uint4* __restrict__ carry, uint32_t* __restrict__ chunk2key) {
@@ -234,21 +178,6 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g2(MsmBatchDev
     }
 }
 
-static __global__ void __launch_bounds__(256) k_msm_unraw_g2(const uint4* __restrict__ raw, g2_xyzz* __restrict__ out, uint64_t n) {
-    constexpr int RW = g2acc29::RAW_WORDS / 4;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = g2acc29::raw_to_xyzz(raw + RW * i);
-}
-
-// raw accumulator states (g1acc29::store_raw) -> R = 2^256 XYZZ points, one
-// lane per point; unwritten slots (empty buckets, chunks without a partial)
-// convert to values nothing reads
-static __global__ void __launch_bounds__(256) k_msm_unraw_g1(const uint4* __restrict__ raw, g1_xyzz* __restrict__ out, uint64_t n) {
-    constexpr int RW = g1acc29::RAW_WORDS / 4;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = g1acc29::raw_to_xyzz(raw + RW * i);
-}
-
 // one base array to convert for the G1 accumulation: count affine points
 // from src into dst, as items [begin, next begin) of the conversion space
 struct MsmConvTask {
@@ -352,12 +281,22 @@ static __global__ void __launch_bounds__(256) k_msm_convert_g2_array(const uint4
 }
 
 // ------------------------------------------------------------------ 6. fix-up
+// The 29-bit accumulator of each group; bucket sums, chunk partials and run
+// sums live in memory as its raw states (RW uint4 per point).
+template <class F> struct Acc29;
+template <> struct Acc29<fq_t> { using A = g1acc29; };
+template <> struct Acc29<fq2_t> { using A = g2acc29; };
+template <class F> using acc29_t = typename Acc29<F>::A;
+template <class F> constexpr int raw_vec() { return acc29_t<F>::RAW_WORDS / 4; }
+
 // hot buckets (more than HOT chunks): one level of an in-place K2-ary tree
 // over the head partials carry[2c] of chunks f+1..l (f, l: first and last
 // chunk of the bucket); afterwards carry[2(f+1)] holds their sum
 template <class F>
 HZK_DEV void hot_tree_chunk(const MsmBatchDev& b, const uint32_t* __restrict__ offs, const uint32_t* __restrict__ chunk2key,
-                            uint32_t stride, xyzz_t<F>* __restrict__ carry, uint32_t ii, uint32_t c) {
+                            uint32_t stride, uint4* __restrict__ carry, uint32_t ii, uint32_t c) {
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
     const MsmInst& d = b.inst[ii];
     if ((uint64_t)c * d.K >= b.stats[4 * ii]) return;
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
@@ -368,20 +307,20 @@ HZK_DEV void hot_tree_chunk(const MsmBatchDev& b, const uint32_t* __restrict__ o
     if (l - f <= HOT) return;
     const uint32_t rel = c - (f + 1), span = l - f;
     if (rel % (stride * K2) != 0) return;
-    xyzz_t<F>* base = carry + 2 * cbeg[ii];
-    xyzz_t<F> acc = base[2 * c];
+    uint4* base = carry + (size_t)RW * 2 * cbeg[ii];
+    A acc = A::load_raw(base + (size_t)RW * 2 * c);
     for (int i = 1; i < K2; ++i) {
         uint32_t r = rel + i * stride;
         if (r >= span) break;
-        acc.add(base[2 * (c + i * stride)]);
+        acc.add(A::load_raw(base + (size_t)RW * 2 * (c + i * stride)));
     }
-    base[2 * c] = acc;
+    acc.store_raw(base + (size_t)RW * 2 * c);
 }
 
 template <class F>
 __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint32_t* __restrict__ offs,
                                                       const uint32_t* __restrict__ chunk2key, uint32_t stride,
-                                                      xyzz_t<F>* __restrict__ carry) {
+                                                      uint4* __restrict__ carry) {
     // Most instances have no hot bucket. Each block lists the instances that
     // do (largest bucket > HOT chunks), in instance order, with the prefix
     // sums of their chunk counts, all in LDS; the grid then strides over the
@@ -414,7 +353,7 @@ __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint3
         for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < cbeg[b.ninst]; gid += step) {
             const uint32_t ii = find_inst_g(cbeg, b.ninst, gid);
             if (b.stats[4 * ii + 2] > HOT * b.inst[ii].K)
-                hot_tree_chunk(b, offs, chunk2key, stride, carry, ii, (uint32_t)(gid - cbeg[ii]));
+                hot_tree_chunk<F>(b, offs, chunk2key, stride, carry, ii, (uint32_t)(gid - cbeg[ii]));
         }
         return;
     }
@@ -430,16 +369,18 @@ __global__ void __launch_bounds__(256) k_msm_hot_tree(MsmBatchDev b, const uint3
             const uint32_t mid = (lo + hi) >> 1;
             if (hbeg[mid] <= g) lo = mid; else hi = mid;
         }
-        hot_tree_chunk(b, offs, chunk2key, stride, carry, hot[lo], (uint32_t)(g - hbeg[lo]));
+        hot_tree_chunk<F>(b, offs, chunk2key, stride, carry, hot[lo], (uint32_t)(g - hbeg[lo]));
     }
 }
 
 // bucket spanning chunks f..l (f < l): bucket = tail(f) + heads(f+1..l), by
 // the lane of chunk f+1 (the first chunk that continues it)
 template <class F>
-__global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
-                                                   const uint32_t* __restrict__ chunk2key,
-                                                   const xyzz_t<F>* __restrict__ carry, xyzz_t<F>* __restrict__ bucket) {
+__global__ void __launch_bounds__(256) k_msm_fixup(MsmBatchDev b, const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ chunk2key, const uint4* __restrict__ carry,
+                                                   uint4* __restrict__ bucket) {
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
     const uint64_t* cbeg = space_g(b, SP_CHUNK);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= cbeg[b.ninst]) return;
@@ -453,14 +394,14 @@ __global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint3
     const uint32_t f = start / d.K;
     if (c != f + 1) return;
     const uint32_t l = (offs[kb + k + 1] - 1) / d.K;
-    const xyzz_t<F>* base = carry + 2 * cbeg[ii];
-    xyzz_t<F> acc = base[2 * f + 1];
+    const uint4* base = carry + (size_t)RW * 2 * cbeg[ii];
+    A acc = A::load_raw(base + (size_t)RW * (2 * f + 1));
     if (l - f > HOT) {
-        acc.add(base[2 * (f + 1)]);
+        acc.add(A::load_raw(base + (size_t)RW * 2 * (f + 1)));
     } else {
-        for (uint32_t cc = f + 1; cc <= l; ++cc) acc.add(base[2 * cc]);
+        for (uint32_t cc = f + 1; cc <= l; ++cc) acc.add(A::load_raw(base + (size_t)RW * 2 * cc));
     }
-    bucket[kb + k] = acc;
+    acc.store_raw(bucket + (size_t)RW * (kb + k));
 }
 
 // ------------------------------------------------------------------ 7. bucket reduction
@@ -469,14 +410,18 @@ __global__ void __launch_bounds__(256, 2) k_msm_fixup(MsmBatchDev b, const uint3
 //     sum_r [ A_r + g * r * R_r ],   A_r = sum_q (q+1) B,   R_r = sum_q B.
 // Stage 7a: lane per (window, run): A_r and R_r by running sums (2g
 // additions, no scalar multiplication). Stage 7b: workgroup per (window,
-// part of Gs = G/S runs): thread t takes m = Gs/256 consecutive runs; the
-// run weights r become t*m + i inside the part, and sum_t t * rr_t (rr_t the
+// part of Gs = G/S runs): thread t takes m = Gs/T consecutive runs; the run
+// weights r become t*m + i inside the part, and sum_t t * rr_t (rr_t the
 // thread's run sum) is the sum over t >= 1 of the suffix sums of rr, one
 // LDS scan. Each part yields its value P_p (weights relative to the part's
-// first run) and its run sum RS_p; the host adds g * p * Gs * RS_p.
+// first run) and its run sum RS_p; the host adds g * p * Gs * RS_p. Buckets,
+// run sums and the LDS slots hold raw accumulator states; only P_p and RS_p
+// are converted to R = 2^256 XYZZ points for the host.
 template <class F>
-__global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xyzz_t<F>* __restrict__ bucket,
-                                                         const uint32_t* __restrict__ offs, xyzz_t<F>* __restrict__ segsum) {
+__global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const uint4* __restrict__ bucket,
+                                                         const uint32_t* __restrict__ offs, uint4* __restrict__ segsum) {
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
     const uint64_t* rbeg = space_g(b, SP_BRED);
     uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= rbeg[b.ninst]) return;
@@ -486,37 +431,39 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const xy
     uint32_t j = t / d.G, sg = t % d.G;
     const uint64_t kb = space_g(b, SP_KEY)[ii];
     const uint32_t* of = offs + kb;
-    const xyzz_t<F>* bk = bucket + kb;
+    const uint4* bk = bucket + (size_t)RW * kb;
     uint32_t kbase = j * d.NB + sg * d.g;
-    xyzz_t<F> run = xyzz_t<F>::inf(), acc = xyzz_t<F>::inf();
+    A run, acc;
+    run.set_inf();
+    acc.set_inf();
     for (int q = (int)d.g - 1; q >= 0; --q) {
         uint32_t key = kbase + q;
-        if (of[key + 1] != of[key]) run.add(bk[key]);
+        if (of[key + 1] != of[key]) run.add(A::load_raw(bk + (size_t)RW * key));
         acc.add(run);
     }
-    segsum[2 * gid] = acc;       // A_r
-    segsum[2 * gid + 1] = run;   // R_r
+    acc.store_raw(segsum + (size_t)RW * 2 * gid);         // A_r
+    run.store_raw(segsum + (size_t)RW * (2 * gid + 1));   // R_r
 }
 
-// Per-thread point slots in LDS, word-major: word w of slot t at
+// Per-thread accumulator slots in LDS, word-major: word w of slot t at
 // sh[w * T + t], so the lanes of a wave touch consecutive words (a point per
-// slot in a row would put every lane's word w in the same bank: 128- or
-// 256-byte strides)
+// slot in a row would put every lane's word w in the same bank)
 template <class F, int T>
 struct LdsPoints {
-    static constexpr int W = sizeof(xyzz_t<F>) / 4;
+    using A = acc29_t<F>;
+    static constexpr int W = A::RAW_WORDS;
     uint32_t* sh;
-    HZK_DEV void put(uint32_t t, const xyzz_t<F>& p) const {
-        const uint32_t* w = (const uint32_t*)&p;
+    HZK_DEV void put(uint32_t t, const A& p) const {
+        uint32_t w[W];
+        p.to_words(w);
 #pragma unroll
         for (int i = 0; i < W; ++i) sh[i * T + t] = w[i];
     }
-    HZK_DEV xyzz_t<F> get(uint32_t t) const {
-        xyzz_t<F> p;
-        uint32_t* w = (uint32_t*)&p;
+    HZK_DEV A get(uint32_t t) const {
+        uint32_t w[W];
 #pragma unroll
         for (int i = 0; i < W; ++i) w[i] = sh[i * T + t];
-        return p;
+        return A::from_words(w);
     }
 };
 
@@ -525,7 +472,7 @@ template <class F, int T>
 HZK_DEV void lds_tree_sum(const LdsPoints<F, T>& sh, uint32_t t) {
     for (uint32_t off = T / 2; off > 0; off >>= 1) {
         if (t < off) {
-            xyzz_t<F> x = sh.get(t);
+            auto x = sh.get(t);
             x.add(sh.get(t + off));
             sh.put(t, x);
         }
@@ -535,11 +482,13 @@ HZK_DEV void lds_tree_sum(const LdsPoints<F, T>& sh, uint32_t t) {
 
 // T threads per part: 256 for single MSMs (shallow: the kernel is latency
 // bound there), 64 in wide batches (each thread takes 4x more runs, so the
-// scan and tree overhead per run is 4x smaller)
+// scan and tree overhead per run is 4x smaller). Dynamic LDS: RAW_WORDS * T words.
 template <class F, int T>
-__global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>* __restrict__ segsum,
+__global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const uint4* __restrict__ segsum,
                                                   xyzz_t<F>* __restrict__ winsum) {
-    __shared__ uint32_t shw[LdsPoints<F, T>::W * T];
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
+    extern __shared__ uint32_t shw[];
     const LdsPoints<F, T> sh{shw};
     const uint64_t* wbeg = space_g(b, SP_WIN);
     uint32_t ii = find_inst_g(wbeg, b.ninst, blockIdx.x);
@@ -548,40 +497,45 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>
     const uint32_t j = local / d.S, part = local % d.S, Gs = d.G / d.S;
     const uint32_t m = Gs >= T ? Gs / T : 1, t = threadIdx.x;
     // runs t*m .. t*m + m - 1 of the part (A_r at 2r, R_r at 2r + 1)
-    const xyzz_t<F>* ss = segsum + 2 * (space_g(b, SP_BRED)[ii] + (size_t)j * d.G + (size_t)part * Gs + (size_t)t * m);
-    xyzz_t<F> a = xyzz_t<F>::inf(), run = xyzz_t<F>::inf(), w = xyzz_t<F>::inf();
+    const uint4* ss =
+        segsum + (size_t)RW * 2 * (space_g(b, SP_BRED)[ii] + (size_t)j * d.G + (size_t)part * Gs + (size_t)t * m);
+    A a, run, w;
+    a.set_inf();
+    run.set_inf();
+    w.set_inf();
     if (t * m < Gs)
         for (int i = (int)m - 1; i >= 0; --i) {
-            a.add(ss[2 * i]);
-            run.add(ss[2 * i + 1]);
+            a.add(A::load_raw(ss + (size_t)RW * 2 * i));
+            run.add(A::load_raw(ss + (size_t)RW * (2 * i + 1)));
             if (i > 0) w.add(run);   // w = sum_i i * R_i
         }
     // inclusive suffix scan of the thread run sums: slot t = sum_{t' >= t} rr_t'
     sh.put(t, run);
     __syncthreads();
     for (uint32_t off = 1; off < T; off <<= 1) {
-        xyzz_t<F> x = sh.get(t);
+        A x = sh.get(t);
         if (t + off < T) x.add(sh.get(t + off));
         __syncthreads();
         sh.put(t, x);
         __syncthreads();
     }
-    const xyzz_t<F> part_sum = sh.get(0);
+    const A part_sum = sh.get(0);
     // P = sum_t a_t + g * sum_t (w_t + m * X_t), X_t = slot t for t >= 1
-    xyzz_t<F> u = t ? sh.get(t) : xyzz_t<F>::inf();
-    for (uint32_t k = 1; k < m; k <<= 1) u = u.dbl();
+    A u = sh.get(t);
+    if (t == 0) u.set_inf();
+    for (uint32_t k = 1; k < m; k <<= 1) u.dbl();
     u.add(w);
     if (T == 256) {
         // shallow: the factor g per thread, one tree
-        for (uint32_t k = 1; k < d.g; k <<= 1) u = u.dbl();
+        for (uint32_t k = 1; k < d.g; k <<= 1) u.dbl();
         u.add(a);
         __syncthreads();
         sh.put(t, u);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
         if (t == 0) {
-            winsum[2 * blockIdx.x] = sh.get(0);     // P_p
-            winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
+            winsum[2 * blockIdx.x] = sh.get(0).to_xyzz();   // P_p
+            winsum[2 * blockIdx.x + 1] = part_sum.to_xyzz();  // RS_p
         }
     } else {
         // less work: both sums first, the factor g once
@@ -589,16 +543,16 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const xyzz_t<F>
         sh.put(t, u);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
-        xyzz_t<F> total = sh.get(0);
+        A total = sh.get(0);
         __syncthreads();
         sh.put(t, a);
         __syncthreads();
         lds_tree_sum<F, T>(sh, t);
         if (t == 0) {
-            for (uint32_t k = 1; k < d.g; k <<= 1) total = total.dbl();
+            for (uint32_t k = 1; k < d.g; k <<= 1) total.dbl();
             total.add(sh.get(0));
-            winsum[2 * blockIdx.x] = total;         // P_p
-            winsum[2 * blockIdx.x + 1] = part_sum;  // RS_p
+            winsum[2 * blockIdx.x] = total.to_xyzz();         // P_p
+            winsum[2 * blockIdx.x + 1] = part_sum.to_xyzz();  // RS_p
         }
     }
 }
@@ -616,13 +570,12 @@ struct MsmBatchLayout {
     // scratch offsets
     size_t o_inst, o_begin, o_stats, o_digits, o_hist, o_count, o_offs, o_tiles, o_sorted, o_coarse, o_chunk2key, o_bucket,
         o_carry, o_segsum, o_win;
-    // G1: base arrays converted for the accumulation, one per distinct source
+    // base arrays converted for the accumulation, one per distinct source
     // array (its largest referenced point count), and which one each instance reads
     std::vector<MsmConvTask> conv;     // dst holds the offset into the scratch until run time
     std::vector<uint32_t> conv_of;     // instance -> task
     uint64_t conv_total = 0;
     size_t o_conv = 0, o_convdesc = 0;
-    size_t o_rawbucket = 0, o_rawcarry = 0;   // G1: raw accumulator states of buckets and chunk partials
 };
 
 template <class F>
@@ -731,11 +684,12 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
     L.o_tiles = take(2 * L.tot[SP_SCANTILE] * 8);
     L.o_sorted = take(L.tot[SP_ENTRY] * 4);
     L.o_coarse = take(L.two_pass ? L.tot[SP_ENTRY] * 8 : 0);
+    const size_t raw = (size_t)raw_vec<F>() * 16;   // bytes per raw accumulator state
     L.o_chunk2key = take(L.tot[SP_CHUNK] * 4);
-    L.o_bucket = take(L.tot[SP_KEY] * sizeof(xyzz_t<F>));
-    L.o_carry = take(2 * L.tot[SP_CHUNK] * sizeof(xyzz_t<F>));
-    L.o_segsum = take(2 * L.tot[SP_BRED] * sizeof(xyzz_t<F>));   // A_r, R_r per run
-    L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));       // P_p, RS_p per part
+    L.o_bucket = take(L.tot[SP_KEY] * raw);
+    L.o_carry = take(2 * L.tot[SP_CHUNK] * raw);             // head and tail partial per chunk
+    L.o_segsum = take(2 * L.tot[SP_BRED] * raw);             // A_r, R_r per run
+    L.o_win = take(2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));   // P_p, RS_p per part (R = 2^256 points, for the host)
     {
         std::map<const void*, uint64_t> need;   // source array -> points referenced (virtual points of level tables)
         for (size_t i = 0; i < count; ++i)
@@ -751,11 +705,8 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         L.conv_of.assign(count, UINT32_MAX);   // UINT32_MAX: bases already converted
         for (size_t i = 0; i < count; ++i)
             if (jobs[i].form == 0) L.conv_of[i] = task_of[L.inst[i].bases];
-        const size_t raw = (g2 ? g2acc29::RAW_WORDS : g1acc29::RAW_WORDS) * 4;
         L.o_conv = take(pts * (g2 ? 128 : 64));
         L.o_convdesc = take(L.conv.size() * sizeof(MsmConvTask));
-        L.o_rawbucket = take(L.tot[SP_KEY] * raw);
-        L.o_rawcarry = take(2 * L.tot[SP_CHUNK] * raw);
     }
     L.bytes = off + 256;
     return L;
@@ -779,44 +730,34 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     sb.sorted = (uint32_t*)(base + L.o_sorted);
     sb.coarse = L.two_pass ? (uint2*)(base + L.o_coarse) : nullptr;
     uint32_t* chunk2key = (uint32_t*)(base + L.o_chunk2key);
-    xyzz_t<F>* bucket = (xyzz_t<F>*)(base + L.o_bucket);
-    xyzz_t<F>* carry = (xyzz_t<F>*)(base + L.o_carry);
-    xyzz_t<F>* segsum = (xyzz_t<F>*)(base + L.o_segsum);
+    uint4* bucket = (uint4*)(base + L.o_bucket);
+    uint4* carry = (uint4*)(base + L.o_carry);
+    uint4* segsum = (uint4*)(base + L.o_segsum);
     xyzz_t<F>* win = (xyzz_t<F>*)(base + L.o_win);
     hipStream_t st = ctx->stream;
 
     if (L.conv_total) {
         TimedLaunch tl(ctx, "msm_convert");
         const uint64_t blocks = (L.conv_total + 255) / 256, cap = (uint64_t)ctx->num_cus * 32;
-        hipLaunchKernelGGL((sizeof(F) == 32 ? k_msm_convert_g1 : k_msm_convert_g2), dim3((unsigned)(blocks < cap ? blocks : cap)),
-                           dim3(256), 0, st, (const MsmConvTask*)(base + L.o_convdesc), (uint32_t)L.conv.size(), L.conv_total);
+        const dim3 grid((unsigned)(blocks < cap ? blocks : cap));
+        const MsmConvTask* tasks = (const MsmConvTask*)(base + L.o_convdesc);
+        if constexpr (sizeof(F) == 32)
+            hipLaunchKernelGGL(k_msm_convert_g1, grid, dim3(256), 0, st, tasks, (uint32_t)L.conv.size(), L.conv_total);
+        else
+            hipLaunchKernelGGL(k_msm_convert_g2, grid, dim3(256), 0, st, tasks, (uint32_t)L.conv.size(), L.conv_total);
     }
     msm_launch_scalar_stages(ctx, b, L.tot, L.max_nb, sb);
     {
         TimedLaunch tl(ctx, "msm_accumulate");
         uint64_t blocks = (L.tot[SP_CHUNK] + 255) / 256;
         uint64_t cap = (uint64_t)ctx->num_cus * 16;   // persistent grid: 16 waves per CU
-        if constexpr (sizeof(F) == 32) {
-            uint4* rb = (uint4*)(base + L.o_rawbucket);
-            uint4* rc = (uint4*)(base + L.o_rawcarry);
-            hipLaunchKernelGGL(k_msm_accumulate_g1, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, rb, rc, chunk2key);
-            const uint64_t nb = L.tot[SP_KEY], nc = 2 * L.tot[SP_CHUNK], c2 = (uint64_t)ctx->num_cus * 32;
-            hipLaunchKernelGGL(k_msm_unraw_g1, dim3((unsigned)std::min<uint64_t>((nb + 255) / 256, c2)), dim3(256), 0, st,
-                               (const uint4*)rb, (g1_xyzz*)bucket, nb);
-            hipLaunchKernelGGL(k_msm_unraw_g1, dim3((unsigned)std::min<uint64_t>((nc + 255) / 256, c2)), dim3(256), 0, st,
-                               (const uint4*)rc, (g1_xyzz*)carry, nc);
-        } else {
-            uint4* rb = (uint4*)(base + L.o_rawbucket);
-            uint4* rc = (uint4*)(base + L.o_rawcarry);
-            hipLaunchKernelGGL(k_msm_accumulate_g2, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, st, b,
-                               (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs, rb, rc, chunk2key);
-            const uint64_t nb = L.tot[SP_KEY], nc = 2 * L.tot[SP_CHUNK], c2 = (uint64_t)ctx->num_cus * 32;
-            hipLaunchKernelGGL(k_msm_unraw_g2, dim3((unsigned)std::min<uint64_t>((nb + 255) / 256, c2)), dim3(256), 0, st,
-                               (const uint4*)rb, (g2_xyzz*)bucket, nb);
-            hipLaunchKernelGGL(k_msm_unraw_g2, dim3((unsigned)std::min<uint64_t>((nc + 255) / 256, c2)), dim3(256), 0, st,
-                               (const uint4*)rc, (g2_xyzz*)carry, nc);
-        }
+        const dim3 grid((unsigned)(blocks < cap ? blocks : cap));
+        if constexpr (sizeof(F) == 32)
+            hipLaunchKernelGGL(k_msm_accumulate_g1, grid, dim3(256), 0, st, b, (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs,
+                               bucket, carry, chunk2key);
+        else
+            hipLaunchKernelGGL(k_msm_accumulate_g2, grid, dim3(256), 0, st, b, (const uint32_t*)sb.sorted, (const uint32_t*)sb.offs,
+                               bucket, carry, chunk2key);
     }
     {
         TimedLaunch tl(ctx, "msm_fixup");
@@ -825,16 +766,18 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
             hipLaunchKernelGGL(k_msm_hot_tree<F>, dim3(std::min<unsigned>(blocks_for(L.tot[SP_CHUNK], 256), 1024)), dim3(256), 0, st, b,
                                (const uint32_t*)sb.offs, (const uint32_t*)chunk2key, stride, carry);
         hipLaunchKernelGGL(k_msm_fixup<F>, dim3(blocks_for(L.tot[SP_CHUNK], 256)), dim3(256), 0, st, b, (const uint32_t*)sb.offs,
-                           (const uint32_t*)chunk2key, (const xyzz_t<F>*)carry, bucket);
+                           (const uint32_t*)chunk2key, (const uint4*)carry, bucket);
     }
     {
         TimedLaunch tl(ctx, "msm_bucket_reduce");
-        hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b,
-                           (const xyzz_t<F>*)bucket, (const uint32_t*)sb.offs, segsum);
+        hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b, (const uint4*)bucket,
+                           (const uint32_t*)sb.offs, segsum);
+        constexpr size_t WW = acc29_t<F>::RAW_WORDS * 4;   // LDS bytes per thread slot
         if (L.regime == MSM_REGIME_BATCH)
-            hipLaunchKernelGGL((k_window_sum<F, 64>), dim3((unsigned)L.tot[SP_WIN]), dim3(64), 0, st, b, (const xyzz_t<F>*)segsum, win);
+            hipLaunchKernelGGL((k_window_sum<F, 64>), dim3((unsigned)L.tot[SP_WIN]), dim3(64), WW * 64, st, b, (const uint4*)segsum, win);
         else
-            hipLaunchKernelGGL((k_window_sum<F, 256>), dim3((unsigned)L.tot[SP_WIN]), dim3(256), 0, st, b, (const xyzz_t<F>*)segsum, win);
+            hipLaunchKernelGGL((k_window_sum<F, 256>), dim3((unsigned)L.tot[SP_WIN]), dim3(256), WW * 256, st, b, (const uint4*)segsum,
+                               win);
     }
 }
 
@@ -874,6 +817,9 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     for (size_t i = 0; i < count; ++i)   // bucket offsets are 32-bit
         HZK_CHECK(ctx, (uint64_t)L.inst[i].n * L.inst[i].W <= 0xffff0000ull, HZK_ERR_UNSUPPORTED,
                   "msm: n=%u with %u windows exceeds 2^32 bucket entries", L.inst[i].n, L.inst[i].W);
+    HZK_CHECK(ctx, (size_t)acc29_t<F>::RAW_WORDS * 4 * 256 <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
+              "msm: the window sums need %zu B of LDS per workgroup, device allows %zu", (size_t)acc29_t<F>::RAW_WORDS * 4 * 256,
+              ctx->lds_per_block);
     HZK_CHECK(ctx, (size_t)L.max_nb * 4 <= ctx->lds_per_block, HZK_ERR_UNSUPPORTED,
               "msm: a %u-bucket window needs %u B of LDS per workgroup, device allows %zu", L.max_nb, L.max_nb * 4,
               ctx->lds_per_block);

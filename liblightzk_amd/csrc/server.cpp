@@ -254,10 +254,17 @@ int stage_msm(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     if (rs.empty()) return 0;
     std::vector<G16Req> reqs;
     for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, r->rs, {}});
+    // the key-and-(r, s) terms of every proof on the host while the device runs the MSMs
+    std::vector<G16Fixed> fx(rs.size());
+    std::thread fixed([&] {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long i = 0; i < (long)rs.size(); ++i) fx[i] = g16_fixed_terms(reqs[i].g, rs[i]->rs);
+    });
     int rc = g16_msm_batch(ctx, reqs.data(), reqs.size(), 0);
+    fixed.join();
     if (rc) return rc;
 #pragma omp parallel for schedule(dynamic, 1)
-    for (long i = 0; i < (long)rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, rs[i]->proof);
+    for (long i = 0; i < (long)rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, fx[i], rs[i]->proof);
     return 0;
 }
 
