@@ -1,6 +1,7 @@
 // C ABI: context, buffers, timing, the device field-arithmetic unit entry,
 // and the MSM entry points (declarations and reference citations in
 // include/hzk.h).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -30,17 +31,22 @@ int ensure_scratch(hzk_ctx* ctx, size_t bytes) {
     }
     if (ctx->scratch_bytes >= bytes) return 0;
     ++ctx->scratch_generation;
+    // Stream-ordered: the old arena is released after the work already
+    // enqueued on this stream, and neither call waits for the device (a
+    // hipFree would stall every stream of the process, e.g. a server's other
+    // pipelines). Growth at least doubles, so a varying batch mix settles
+    // after a few steps.
+    const size_t old = ctx->scratch_bytes;
     if (ctx->scratch) {
-        HZK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        HZK_HIP(ctx, hipFree(ctx->scratch));
+        HZK_HIP(ctx, hipFreeAsync(ctx->scratch, ctx->stream));
         ctx->scratch = nullptr;
         ctx->scratch_bytes = 0;
     }
-    size_t want = bytes + bytes / 8;
-    hipError_t e = hipMalloc(&ctx->scratch, want);
+    size_t want = std::max(bytes + bytes / 8, 2 * old);
+    hipError_t e = hipMallocAsync(&ctx->scratch, want, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        e = hipMalloc(&ctx->scratch, bytes);
+        e = hipMallocAsync(&ctx->scratch, bytes, ctx->stream);
         want = bytes;
     }
     if (e != hipSuccess) {
@@ -62,10 +68,11 @@ int stage_acquire(hzk_ctx* ctx, size_t bytes, char** buf, int* slot) {
     }
     if (!st.done) HZK_HIP(ctx, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
     if (st.bytes < bytes) {
+        const size_t old = st.bytes;
         if (st.p) HZK_HIP(ctx, hipHostFree(st.p));
         st.p = nullptr;
         st.bytes = 0;
-        size_t want = bytes < 65536 ? 65536 : bytes + bytes / 4;
+        size_t want = std::max<size_t>({(size_t)1 << 20, bytes + bytes / 4, 2 * old});   // pinned allocations are slow: grow rarely
         HZK_HIP(ctx, hipHostMalloc(&st.p, want, hipHostMallocDefault));
         st.bytes = want;
     }
@@ -186,6 +193,14 @@ int hzk_ctx_create(int device, void* stream, hzk_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return HZK_ERR_HIP; }
     if (msm_noalloc_init(device) != HZK_OK) { delete ctx; return HZK_ERR_HIP; }
+    {   // scratch arenas come from the device's stream-ordered pool; freed ones stay cached there
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        (void)hipGetLastError();
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         ctx->lds_per_block = prop.sharedMemPerBlock;
@@ -210,7 +225,10 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
     for (auto& kv : ctx->graphs)
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->scratch) {
+        (void)hipFreeAsync(ctx->scratch, ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
     ntt_release(ctx);
     if (ctx->desc) (void)hipFree(ctx->desc);
     if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);

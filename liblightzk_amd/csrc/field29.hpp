@@ -183,6 +183,69 @@ struct fq29 {
 #pragma unroll
         for (int c = 0; c < N; ++c) r[c].l[8] = (uint32_t)acc[c];
     }
+    // N independent squarings, interleaved as in mont_mul_n: the column sum
+    // sum_j a_j a_(k-j) taken as 2 sum_(j < k-j) a_j a_(k-j) + a_(k/2)^2 (the
+    // same integer, so the same result) with the doubled limbs precomputed:
+    // 45 limb products instead of 81. Inputs normalized (limbs below 2^29,
+    // doubled below 2^30; a column stays below 2^62.2).
+    template <int N>
+    static HZK_DEV void mont_sqr_n(fq29* r, const fq29* a) {
+        uint32_t m[N][9], d[N][9];
+        uint64_t acc[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            acc[c] = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) d[c][i] = a[c].l[i] << 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 17; ++k) {
+            if (k < 9) {
+#pragma unroll
+                for (int j = 0; j < k; ++j)
+#pragma unroll
+                    for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+            }
+#pragma unroll
+            for (int j = (k > 8 ? k - 8 : 0); 2 * j < k; ++j)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)d[c][j] * a[c].l[k - j];
+            if ((k & 1) == 0)
+#pragma unroll
+                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[k / 2] * a[c].l[k / 2];
+            if (k < 9) {
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    m[c][k] = ((uint32_t)acc[c] * M0) & MASK;
+                    acc[c] += (uint64_t)m[c][k] * P[0];
+                    acc[c] >>= 29;
+                }
+            } else {
+#pragma unroll
+                for (int j = k - 8; j < 9; ++j)
+#pragma unroll
+                    for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    r[c].l[k - 9] = (uint32_t)acc[c] & MASK;
+                    acc[c] >>= 29;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c) r[c].l[8] = (uint32_t)acc[c];
+    }
+    static HZK_DEV void sqr2(fq29& r0, const fq29& a0, fq29& r1, const fq29& a1) {
+        const fq29 a[2] = {a0, a1};
+        fq29 r[2];
+        mont_sqr_n<2>(r, a);
+        r0 = r[0], r1 = r[1];
+    }
+    static HZK_DEV fq29 sqr1(const fq29& a0) {
+        fq29 r;
+        mont_sqr_n<1>(&r, &a0);
+        return r;
+    }
     static HZK_DEV void mul2(fq29& r0, const fq29& a0, const fq29& b0, fq29& r1, const fq29& a1, const fq29& b1) {
         const fq29 a[2] = {a0, a1}, b[2] = {b0, b1};
         fq29 r[2];
@@ -299,12 +362,12 @@ struct g1acc29 {
         if (P.is_zero_mod_p()) {
             if (!R.is_zero_mod_p()) { set_inf(); return; }
             P = (Y + Y).norm();                          // 2Y < 8
-            const fq29 xx = X * X;                       // 8 x 8
+            const fq29 xx = fq29::sqr1(X);                     // 8 x 8
             R = (xx + xx + xx).norm();                   // 3X^2 < 6
             dbl = true;
         }
         fq29 PP, RR;
-        fq29::mul2(PP, P, P, RR, R, R);                  // 10 x 10, 6 x 6
+        fq29::sqr2(PP, P, RR, R);                  // 10 x 10, 6 x 6
         fq29 PPP, Q, ZZ3;
         fq29::mul3(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);   // 10 x 2, 8 x 2, 2 x 2
         fq29 T = Q + Q;
@@ -376,7 +439,7 @@ struct g1acc29 {
         if (P.is_zero_mod_p()) {
             if (!R.is_zero_mod_p()) { set_inf(); return; }
             P = (Y + Y).norm();                                      // 2Y < 8
-            const fq29 xx = X * X;                                   // 8 x 8
+            const fq29 xx = fq29::sqr1(X);                                 // 8 x 8
             R = (xx + xx + xx).norm();                               // 3X^2 < 6
             U1 = X, S1 = Y, ZZ12 = ZZ, ZZZ12 = ZZZ;
             d = true;
@@ -385,14 +448,14 @@ struct g1acc29 {
     }
     HZK_DEV void dbl() {
         if (inf) return;
-        const fq29 xx = X * X;
+        const fq29 xx = fq29::sqr1(X);
         finish((Y + Y).norm(), (xx + xx + xx).norm(), X, Y, ZZ, ZZZ, true);
     }
     // the common tail: P < 8, R < 6, U1 < 8, S1 < 4, ZZ12, ZZZ12 < 2
     HZK_DEV void finish(const fq29& P, const fq29& R, const fq29& U1, const fq29& S1, const fq29& ZZ12, const fq29& ZZZ12,
                         bool d) {
         fq29 PP, RR;
-        fq29::mul2(PP, P, P, RR, R, R);                              // 8 x 8, 6 x 6
+        fq29::sqr2(PP, P, RR, R);                              // 8 x 8, 6 x 6
         fq29 PPP, Q, ZZn;
         fq29::mul3(PPP, P, PP, Q, U1, PP, ZZn, ZZ12, PP);            // 8 x 2, 8 x 2, 2 x 2
         fq29 T = Q + Q;
@@ -464,26 +527,26 @@ struct g2acc29 {
     // registers)
     HZK_DEV void add_affine(const fq2_29& x, const fq2_29& y) {
         if (inf) { set_affine(x, y); return; }
-        fq2_29 P = fq2_29::sub(x * ZZ, X, fq29::P2).reduce();                // U2 = x ZZ: (2)(10); P < 2
-        fq2_29 R = fq2_29::sub(y * ZZZ, Y, fq29::P2).reduce();               // S2 = y ZZZ; R < 2
+        fq2_29 P = fq2_29::sub(fq2_29::mul_x(x, ZZ), X, fq29::P2).reduce();                // U2 = x ZZ: (2)(10); P < 2
+        fq2_29 R = fq2_29::sub(fq2_29::mul_x(y, ZZZ), Y, fq29::P2).reduce();               // S2 = y ZZZ; R < 2
         bool dbl = false;
         if (P.is_zero_mod_p()) {
             if (!R.is_zero_mod_p()) { set_inf(); return; }
             P = (Y + Y).norm().reduce();                                     // 2Y < 2
-            const fq2_29 xx = X.sqr();
+            const fq2_29 xx = X.sqr_x();
             R = (xx + xx + xx).norm().reduce();                              // 3X^2 < 2
             dbl = true;
         }
-        const fq2_29 PP = P.sqr();                                           // < 4
-        const fq2_29 PPP = P * PP;                                           // (4)(6), < 6
-        ZZ = ZZ * PP;                                                        // (10)(6)
-        const fq2_29 Q = X * PP;                                             // (4)(6), < 6
+        const fq2_29 PP = P.sqr_x();                                           // < 4
+        const fq2_29 PPP = fq2_29::mul_x(P, PP);                                           // (4)(6), < 6
+        ZZ = fq2_29::mul_x(ZZ, PP);                                                        // (10)(6)
+        const fq2_29 Q = fq2_29::mul_x(X, PP);                                             // (4)(6), < 6
         fq2_29 T = Q + Q;
         if (!dbl) T = T + PPP;
-        X = fq2_29::sub(R.sqr(), T, fq29::P18).reduce();                     // RR - (PPP + 2Q) + 18p, then < 2
-        const fq2_29 t = R * fq2_29::sub(Q, X, fq29::P2).reduce();           // R (Q - X3): (4)(4)
-        Y = fq2_29::sub(t, Y * PPP, fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
-        ZZZ = ZZZ * PPP;                                                     // (10)(10)
+        X = fq2_29::sub(R.sqr_x(), T, fq29::P18).reduce();                     // RR - (PPP + 2Q) + 18p, then < 2
+        const fq2_29 t = fq2_29::mul_x(R, fq2_29::sub(Q, X, fq29::P2).reduce());           // R (Q - X3): (4)(4)
+        Y = fq2_29::sub(t, fq2_29::mul_x(Y, PPP), fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
+        ZZZ = fq2_29::mul_x(ZZZ, PPP);                                                     // (10)(10)
     }
     // raw state: 72 words, infinity has ZZ = 0 (read back by load_raw)
     static constexpr int RAW_WORDS = 72;

@@ -485,6 +485,24 @@ int hzk_server_start(hzk_server* s) {
         (void)hipMemGetInfo(&fr, &tot);
         s->budget = fr / 2;
     }
+    // pinned witness buffers up front (an equal share of the cap per circuit,
+    // at most what the pipelines can hold in flight): allocating pinned memory
+    // while requests flow serializes with the device work of the pipelines
+    (void)hipSetDevice(s->device);
+    const size_t ncirc = s->circuits.size(), in_flight = 2 * (size_t)s->max_batch * s->pipes.size();
+    for (size_t c = 0; c < ncirc; ++c) {
+        const size_t zbytes = s->circuits[c]->num_vars * 32;
+        const size_t want = std::min(in_flight, PINNED_CAP / ncirc / zbytes);
+        while (s->pinned_free[c].size() < want && s->pinned_bytes + zbytes <= PINNED_CAP) {
+            void* p = nullptr;
+            if (hipHostMalloc(&p, zbytes, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                break;
+            }
+            s->pinned_free[c].push_back(p);
+            s->pinned_bytes += zbytes;
+        }
+    }
     s->running = true;
     for (auto& p : s->pipes) p->th = std::thread(worker_loop, s, p.get());
     return HZK_OK;
