@@ -65,6 +65,7 @@ struct MsmInst {
     const void* bases;     // affine points
     uint32_t n, c, W, NB, nk, K, g, G, T, tile;
     uint32_t S;            // window-sum workgroups per window
+    uint32_t fused;        // 1: each window-sum thread reduces one run of g buckets itself (no run-sum pass)
     // Level tables: nd scalars with Wd digit windows over a table of Lg levels
     // (point t*nd + i = 2^(c t) P_i, t < Lg); each bucket set ("window" of the
     // pipeline) serves Lg consecutive digit windows, so W = ceil(Wd / Lg) and

@@ -427,6 +427,7 @@ __global__ void __launch_bounds__(128) k_bucket_segments(MsmBatchDev b, const ui
     if (gid >= rbeg[b.ninst]) return;
     uint32_t ii = find_inst_g(rbeg, b.ninst, gid);
     const MsmInst& d = b.inst[ii];
+    if (d.fused) return;   // its window-sum kernel reduces the runs itself
     uint32_t t = (uint32_t)(gid - rbeg[ii]);
     uint32_t j = t / d.G, sg = t % d.G;
     const uint64_t kb = space_g(b, SP_KEY)[ii];
@@ -485,6 +486,7 @@ HZK_DEV void lds_tree_sum(const LdsPoints<F, T>& sh, uint32_t t) {
 // scan and tree overhead per run is 4x smaller). Dynamic LDS: RAW_WORDS * T words.
 template <class F, int T>
 __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const uint4* __restrict__ segsum,
+                                                  const uint4* __restrict__ bucket, const uint32_t* __restrict__ offs,
                                                   xyzz_t<F>* __restrict__ winsum) {
     using A = acc29_t<F>;
     constexpr int RW = raw_vec<F>();
@@ -503,12 +505,25 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const uint4* __
     a.set_inf();
     run.set_inf();
     w.set_inf();
-    if (t * m < Gs)
+    if (d.fused) {
+        // m = 1: this lane's run r = part * T + t, buckets r*g .. r*g + g-1 of
+        // window j; A_r and R_r by running sums as in k_bucket_segments
+        const uint64_t kb = space_g(b, SP_KEY)[ii];
+        const uint32_t* of = offs + kb;
+        const uint4* bk = bucket + (size_t)RW * kb;
+        const uint32_t kbase = j * d.NB + (part * T + t) * d.g;
+        for (int q = (int)d.g - 1; q >= 0; --q) {
+            const uint32_t key = kbase + q;
+            if (of[key + 1] != of[key]) run.add(A::load_raw(bk + (size_t)RW * key));
+            a.add(run);
+        }
+    } else if (t * m < Gs) {
         for (int i = (int)m - 1; i >= 0; --i) {
             a.add(A::load_raw(ss + (size_t)RW * 2 * i));
             run.add(A::load_raw(ss + (size_t)RW * (2 * i + 1)));
             if (i > 0) w.add(run);   // w = sum_i i * R_i
         }
+    }
     // inclusive suffix scan of the thread run sums: slot t = sum_{t' >= t} rr_t'
     sh.put(t, run);
     __syncthreads();
@@ -637,6 +652,13 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         }
         d.S = 1;   // window-sum partials: at most ~2 run sums per lane of 256
         while (d.S < 64 && d.G / d.S > 512) d.S *= 2;
+        // single MSMs: one run per lane of 256, reduced from the buckets inside
+        // the window-sum kernel (2g + 20 dependent additions per lane instead
+        // of 2g in a run-sum pass and then up to 28), while the W * G / 256
+        // workgroups of four waves fit one wave per SIMD (a second round of
+        // waves would double the kernel's latency-bound time)
+        d.fused = regime == MSM_REGIME_SINGLE && d.G % 256 == 0 && (uint64_t)d.W * (d.G / 256) <= 256;
+        if (d.fused) d.S = d.G / 256;
         uint64_t entries = (uint64_t)d.n * d.W;
         uint64_t sz[SP_COUNT];
         sz[SP_DIGIT] = d.nd;
@@ -770,14 +792,18 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
     }
     {
         TimedLaunch tl(ctx, "msm_bucket_reduce");
-        hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b, (const uint4*)bucket,
-                           (const uint32_t*)sb.offs, segsum);
+        bool all_fused = true;
+        for (const MsmInst& d : L.inst) all_fused = all_fused && d.fused;
+        if (!all_fused)
+            hipLaunchKernelGGL(k_bucket_segments<F>, dim3(blocks_for(L.tot[SP_BRED], 128)), dim3(128), 0, st, b, (const uint4*)bucket,
+                               (const uint32_t*)sb.offs, segsum);
         constexpr size_t WW = acc29_t<F>::RAW_WORDS * 4;   // LDS bytes per thread slot
         if (L.regime == MSM_REGIME_BATCH)
-            hipLaunchKernelGGL((k_window_sum<F, 64>), dim3((unsigned)L.tot[SP_WIN]), dim3(64), WW * 64, st, b, (const uint4*)segsum, win);
+            hipLaunchKernelGGL((k_window_sum<F, 64>), dim3((unsigned)L.tot[SP_WIN]), dim3(64), WW * 64, st, b, (const uint4*)segsum,
+                               (const uint4*)bucket, (const uint32_t*)sb.offs, win);
         else
             hipLaunchKernelGGL((k_window_sum<F, 256>), dim3((unsigned)L.tot[SP_WIN]), dim3(256), WW * 256, st, b, (const uint4*)segsum,
-                               win);
+                               (const uint4*)bucket, (const uint32_t*)sb.offs, win);
     }
 }
 
