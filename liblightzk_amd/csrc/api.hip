@@ -58,6 +58,27 @@ int ensure_scratch(hzk_ctx* ctx, size_t bytes) {
     return 0;
 }
 
+int upload_desc(hzk_ctx* ctx, const void* host, size_t bytes, const void** dev) {
+    if (ctx->desc_bytes < bytes) {   // stream-ordered like the scratch arena: no device-wide stall
+        if (ctx->desc) HZK_HIP(ctx, hipFreeAsync(ctx->desc, ctx->stream));
+        ctx->desc = nullptr;
+        ctx->desc_bytes = 0;
+        const size_t want = std::max<size_t>({bytes * 2, (size_t)1 << 16});
+        HZK_HIP(ctx, hipMallocAsync(&ctx->desc, want, ctx->stream));
+        ctx->desc_bytes = want;
+        ++ctx->desc_generation;
+    }
+    char* pin;
+    int slot;
+    int rc = stage_acquire(ctx, bytes, &pin, &slot);
+    if (rc) return rc;
+    std::memcpy(pin, host, bytes);
+    HZK_HIP(ctx, hipMemcpyAsync(ctx->desc, pin, bytes, hipMemcpyHostToDevice, ctx->stream));
+    rc = stage_fence(ctx, slot);
+    *dev = ctx->desc;
+    return rc;
+}
+
 int stage_acquire(hzk_ctx* ctx, size_t bytes, char** buf, int* slot) {
     int i = ctx->stage_next;
     ctx->stage_next = (i + 1) % hzk_ctx::STAGE_SLOTS;
@@ -230,7 +251,10 @@ void hzk_ctx_destroy(hzk_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
     }
     ntt_release(ctx);
-    if (ctx->desc) (void)hipFree(ctx->desc);
+    if (ctx->desc) {
+        (void)hipFreeAsync(ctx->desc, ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
     if (ctx->comb_g1) (void)hipFree(ctx->comb_g1);
     if (ctx->comb_g2) (void)hipFree(ctx->comb_g2);
     if (ctx->conv) (void)hipFree(ctx->conv);

@@ -118,6 +118,34 @@ int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C,
 int gather_rows(hzk_ctx* ctx, const void* src, const uint32_t* idx, size_t n, size_t elem_bytes, void* dst);
 // dst[k] = z[idx[k]] * r over Fr (Montgomery; r: 32 bytes on the host), k < n
 int gather_scale_fr(hzk_ctx* ctx, const void* z, const uint32_t* idx, size_t n, const void* r_host, void* dst);
+// Copy `bytes` of host descriptors into ctx->desc (grown stream-ordered when
+// short) through pinned staging, on ctx's stream; *dev points at them there.
+// The next upload reuses the buffer: enqueue the kernels that read these first.
+int upload_desc(hzk_ctx* ctx, const void* host, size_t bytes, const void** dev);
+
+// The per-request vector stages of several proofs, one launch each (spmv.hip).
+struct SpmvJob {   // Az, Bz, Cz = A z, B z, C z for one request
+    const hzk_csr* m[3];
+    const void* z;
+    void* out[3];
+};
+int spmv3_batch(hzk_ctx* ctx, const SpmvJob* jobs, size_t count);
+struct AbcJob {    // A <- (A o B - C) * scale over n elements
+    void* A;
+    const void* B;
+    const void* C;
+    const void* scale;   // device Fr or null
+    size_t n;
+};
+int pointwise_abc_batch(hzk_ctx* ctx, const AbcJob* jobs, size_t count);
+struct GatherJob { // dst[k] = src[idx[k]] (idx null: src[k]) for n elements of elem_bytes; times r (Fr) when scale
+    const void* src;
+    const uint32_t* idx;
+    void* dst;
+    size_t n, elem_bytes;
+    const void* r_host;  // 32 bytes, Montgomery, or null; only with elem_bytes = 32
+};
+int gather_batch(hzk_ctx* ctx, const GatherJob* jobs, size_t count);
 // in-place standard NTT (inverse: w^-1 and 1/n) and coset NTT over Fr (ntt.hip)
 int ntt_standard_raw(hzk_ctx* ctx, void* x, size_t deg, int inverse);
 struct NttJob {

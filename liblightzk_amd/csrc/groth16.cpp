@@ -377,11 +377,19 @@ namespace hzk {
 
 int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n) {
     std::vector<NttJob> jobs;
-    for (size_t i = 0; i < n; ++i) {
-        hzk_g16* g = reqs[i].g;
-        hzk_csr cs[3];
-        for (int m = 0; m < 3; ++m) cs[m] = {g->d_val[m], g->d_col[m], g->d_rp[m], g->M, g->num_vars, g->m[m].col.size()};
-        int rc = spmv3(ctx, &cs[0], &cs[1], &cs[2], reqs[i].z, reqs[i].a, reqs[i].b, reqs[i].c);
+    {   // the three SpMVs of every request: one launch
+        std::vector<hzk_csr> cs(3 * n);
+        std::vector<SpmvJob> sj(n);
+        for (size_t i = 0; i < n; ++i) {
+            hzk_g16* g = reqs[i].g;
+            for (int m = 0; m < 3; ++m) {
+                cs[3 * i + m] = {g->d_val[m], g->d_col[m], g->d_rp[m], g->M, g->num_vars, g->m[m].col.size()};
+                sj[i].m[m] = &cs[3 * i + m];
+            }
+            sj[i].z = reqs[i].z;
+            sj[i].out[0] = reqs[i].a, sj[i].out[1] = reqs[i].b, sj[i].out[2] = reqs[i].c;
+        }
+        int rc = spmv3_batch(ctx, sj.data(), n);
         if (rc) return rc;
     }
     // evaluations on <w> (natural order) -> coefficients (bit-reversed, times M)
@@ -396,34 +404,40 @@ int g16_h_batch(hzk_ctx* ctx, G16Req* reqs, size_t n) {
         if (rc) return rc;
     }
     jobs.clear();
+    std::vector<AbcJob> aj(n);
     for (size_t i = 0; i < n; ++i) {
-        int rc = pointwise_abc(ctx, reqs[i].g->M, reqs[i].a, reqs[i].b, reqs[i].c, reqs[i].g->zinv);
-        if (rc) return rc;
+        aj[i] = {reqs[i].a, reqs[i].b, reqs[i].c, reqs[i].g->zinv, reqs[i].g->M};
         jobs.push_back({reqs[i].a, reqs[i].g->M, 6});
     }
+    int rc = pointwise_abc_batch(ctx, aj.data(), n);
+    if (rc) return rc;
     return ntt_batch(ctx, jobs.data(), jobs.size(), 0);
 }
 
 int g16_msm_batch(hzk_ctx* ctx, G16Req* reqs, size_t n, int graph) {
     std::vector<MsmJob> j1, j2;
+    std::vector<GatherJob> gj;
     for (size_t i = 0; i < n; ++i) {
         hzk_g16* g = reqs[i].g;
         const size_t N1 = g->num_vars, l = g->num_public, nL = N1 - l - 1, nH = g->M - 1;
         char* r = reqs[i].res;
-        // scalars into the buffers the H stage no longer needs:
-        // b = z_A | z_B, c = z_{l+1..} | h | r z_B
+        // scalars into the buffers the H stage no longer needs (one launch for
+        // every request): b = z_A | z_B, c = z_{l+1..} | h | r z_B
         char* b = (char*)reqs[i].b;
         char* c = (char*)reqs[i].c;
-        int rc = gather_rows(ctx, reqs[i].z, g->idxA, g->nA, 32, b);
-        if (!rc) rc = gather_rows(ctx, reqs[i].z, g->idxB, g->nB, 32, b + g->nA * 32);
-        if (!rc) rc = gather_scale_fr(ctx, reqs[i].z, g->idxB, g->nB, reqs[i].rs, c + (nL + nH) * 32);
-        if (rc) return rc;
-        HZK_HIP(ctx, hipMemcpyAsync(c, (char*)reqs[i].z + (l + 1) * 32, nL * 32, hipMemcpyDeviceToDevice, ctx->stream));
-        HZK_HIP(ctx, hipMemcpyAsync(c + nL * 32, reqs[i].a, nH * 32, hipMemcpyDeviceToDevice, ctx->stream));
+        gj.push_back({reqs[i].z, g->idxA, b, g->nA, 32, nullptr});
+        gj.push_back({reqs[i].z, g->idxB, b + g->nA * 32, g->nB, 32, nullptr});
+        gj.push_back({reqs[i].z, g->idxB, c + (nL + nH) * 32, g->nB, 32, reqs[i].rs});
+        gj.push_back({(const char*)reqs[i].z + (l + 1) * 32, nullptr, c, nL, 32, nullptr});
+        gj.push_back({reqs[i].a, nullptr, c + nL * 32, nH, 32, nullptr});
         const uint32_t wc = g->lv > 1 ? (uint32_t)g->lc : 0, L = g->lv > 1 ? g->lv : 0;
         j1.push_back({b, g->lv > 1 ? g->A1L : g->A1cv, g->nA, r, wc, L, 1});
         j1.push_back({c, g->lv > 1 ? g->C1L : g->C1v, g->nC, r + 96, wc, L, 1});
         j2.push_back({b + g->nA * 32, g->lv > 1 ? g->B2L : g->B2cv, g->nB, r + 192, wc, L, 1});
+    }
+    {
+        int rc = gather_batch(ctx, gj.data(), gj.size());
+        if (rc) return rc;
     }
     // the G2 batch runs on the auxiliary context's stream, concurrently with
     // the G1 batch: the latency-bound tails of one (bucket reduction, window

@@ -647,26 +647,16 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     }
     // descriptors to the device
     size_t pbytes = pds.size() * sizeof(NttPassDesc), ebytes = eds.size() * sizeof(NttElemDesc);
-    size_t need = ((pbytes + 255) & ~size_t(255)) + ebytes + 256;
-    if (ctx->desc_bytes < need) {
-        if (ctx->desc) { HZK_HIP(ctx, hipStreamSynchronize(ctx->stream)); HZK_HIP(ctx, hipFree(ctx->desc)); ctx->desc = nullptr; }
-        HZK_HIP(ctx, hipMalloc(&ctx->desc, need * 2));
-        ctx->desc_bytes = need * 2;
-        ++ctx->desc_generation;
-    }
-    char* pin;
-    int slot;
-    int rc = stage_acquire(ctx, need, &pin, &slot);
-    if (rc) return rc;
     size_t eoff = (pbytes + 255) & ~size_t(255);
-    std::memcpy(pin, pds.data(), pbytes);
-    std::memcpy(pin + eoff, eds.data(), ebytes);
-    hipStream_t st = ctx->stream;
-    HZK_HIP(ctx, hipMemcpyAsync(ctx->desc, pin, eoff + ebytes, hipMemcpyHostToDevice, st));
-    rc = stage_fence(ctx, slot);
+    std::vector<char> host(eoff + ebytes);
+    std::memcpy(host.data(), pds.data(), pbytes);
+    std::memcpy(host.data() + eoff, eds.data(), ebytes);
+    const void* dev = nullptr;
+    int rc = upload_desc(ctx, host.data(), host.size(), &dev);
     if (rc) return rc;
-    const NttPassDesc* dpd = (const NttPassDesc*)ctx->desc;
-    const NttElemDesc* ded = (const NttElemDesc*)((char*)ctx->desc + eoff);
+    hipStream_t st = ctx->stream;
+    const NttPassDesc* dpd = (const NttPassDesc*)dev;
+    const NttElemDesc* ded = (const NttElemDesc*)((const char*)dev + eoff);
 
     auto stages = [&]() {
         for (size_t i = 0; i < count; ++i) {   // 2- and 4-point transforms

@@ -10,7 +10,9 @@
 // One lane per row over the three matrices in one launch (synthetic R1CS rows
 // hold 1-3 nonzeros, so a row is a handful of gathers and Fr products; SpMV is
 // gather-bound, ~68 B per nonzero).
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include "common.hpp"
 #include "field.hpp"
 
@@ -117,6 +119,156 @@ int pointwise_abc(hzk_ctx* ctx, size_t n, void* A, const void* B, const void* C,
                        (const fr_t*)scale);
     HZK_HIP(ctx, hipGetLastError());
     return HZK_OK;
+}
+
+// ------------------------------------------------------------------ batched forms
+// The H stage and the MSM scalar preparation of a server step touch every
+// request's vectors with the same few operations; each operation runs as one
+// launch over a descriptor table (lane space = the concatenation of the
+// requests' lanes, a lane finds its request by binary search on `begin`).
+
+template <class D>
+HZK_DEV uint32_t find_task(const D* __restrict__ t, uint32_t nt, uint64_t gid) {
+    uint32_t lo = 0, hi = nt;   // t[lo].begin <= gid < t[hi].begin
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (t[mid].begin <= gid) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct SpmvTask {
+    CsrDev m[3];
+    const fr_t* z;
+    fr_t* out[3];
+    uint64_t rows, begin;   // lanes: 3 * rows (matrix-major), rows = the largest row count
+};
+
+__global__ void __launch_bounds__(256) k_spmv3_batch(const SpmvTask* __restrict__ tasks, uint32_t nt, uint64_t total) {
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        const SpmvTask& t = tasks[find_task(tasks, nt, gid)];
+        const uint64_t local = gid - t.begin, sel = local / t.rows, r = local - sel * t.rows;
+        const CsrDev& m = t.m[sel];
+        if (r >= m.row) continue;
+        const uint64_t b = m.row_ptr[r], e = r + 1 < m.row ? m.row_ptr[r + 1] : m.cnt;
+        fr_t acc = fr_t::zero();
+        for (uint64_t j = b; j < e; ++j) acc += t.z[m.col[j]] * m.val[j];
+        t.out[sel][r] = acc;
+    }
+}
+
+struct AbcTask {
+    fr_t* A;
+    const fr_t* B;
+    const fr_t* C;
+    const fr_t* scale;
+    uint64_t begin;
+};
+
+__global__ void __launch_bounds__(256) k_abc_batch(const AbcTask* __restrict__ tasks, uint32_t nt, uint64_t total) {
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        const AbcTask& t = tasks[find_task(tasks, nt, gid)];
+        const uint64_t i = gid - t.begin;
+        fr_t v = t.A[i] * t.B[i] - t.C[i];
+        if (t.scale) v = v * *t.scale;
+        t.A[i] = v;
+    }
+}
+
+struct GatherTask {
+    const uint4* src;
+    const uint32_t* idx;   // null: contiguous
+    uint4* dst;
+    uint64_t begin;        // lanes: n * q words, or n elements when scaled
+    fr_t r;
+    uint32_t q, scaled;
+};
+
+__global__ void __launch_bounds__(256) k_gather_batch(const GatherTask* __restrict__ tasks, uint32_t nt, uint64_t total) {
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += (uint64_t)gridDim.x * blockDim.x) {
+        const GatherTask& t = tasks[find_task(tasks, nt, gid)];
+        const uint64_t local = gid - t.begin;
+        if (t.scaled) {   // one Fr element per lane
+            const uint64_t s = t.idx ? t.idx[local] : local;
+            ((fr_t*)t.dst)[local] = ((const fr_t*)t.src)[s] * t.r;
+        } else {          // one 16-byte word per lane: coalesced writes
+            const uint64_t e = local / t.q, w = local - e * t.q;
+            const uint64_t s = t.idx ? t.idx[e] : e;
+            t.dst[local] = t.src[s * t.q + w];
+        }
+    }
+}
+
+template <class Task>
+static int launch_tasks(hzk_ctx* ctx, const std::vector<Task>& tasks, uint64_t total, void (*kern)(const Task*, uint32_t, uint64_t),
+                        const char* name) {
+    if (tasks.empty() || total == 0) return HZK_OK;
+    const void* dev = nullptr;
+    int rc = upload_desc(ctx, tasks.data(), tasks.size() * sizeof(Task), &dev);
+    if (rc) return rc;
+    const uint64_t blocks = (total + 255) / 256, cap = (uint64_t)ctx->num_cus * 32;
+    TimedLaunch tl(ctx, name);
+    hipLaunchKernelGGL(kern, dim3((unsigned)(blocks < cap ? blocks : cap)), dim3(256), 0, ctx->stream, (const Task*)dev,
+                       (uint32_t)tasks.size(), total);
+    HZK_HIP(ctx, hipGetLastError());
+    return HZK_OK;
+}
+
+int spmv3_batch(hzk_ctx* ctx, const SpmvJob* jobs, size_t count) {
+    std::vector<SpmvTask> tasks;
+    uint64_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        HZK_CHECK(ctx, jobs[i].z && jobs[i].out[0] && jobs[i].out[1] && jobs[i].out[2], HZK_ERR_INVALID, "spmv3: null argument");
+        SpmvTask t{};
+        t.z = (const fr_t*)jobs[i].z;
+        for (int k = 0; k < 3; ++k) {
+            const hzk_csr* m = jobs[i].m[k];
+            HZK_CHECK(ctx, m && (m->row == 0 || (m->row_ptr && (m->cnt == 0 || (m->val && m->col_idx)))), HZK_ERR_INVALID,
+                      "spmv3: matrix %d has null arrays", k);
+            t.m[k] = {(const fr_t*)m->val, m->col_idx, m->row_ptr, m->row, m->cnt};
+            t.out[k] = (fr_t*)jobs[i].out[k];
+            t.rows = std::max<uint64_t>(t.rows, m->row);
+        }
+        if (!t.rows) continue;
+        t.begin = total;
+        total += 3 * t.rows;
+        tasks.push_back(t);
+    }
+    return launch_tasks(ctx, tasks, total, k_spmv3_batch, "spmv3");
+}
+
+int pointwise_abc_batch(hzk_ctx* ctx, const AbcJob* jobs, size_t count) {
+    std::vector<AbcTask> tasks;
+    uint64_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        HZK_CHECK(ctx, jobs[i].A && jobs[i].B && jobs[i].C, HZK_ERR_INVALID, "pointwise_abc: null argument");
+        if (!jobs[i].n) continue;
+        tasks.push_back({(fr_t*)jobs[i].A, (const fr_t*)jobs[i].B, (const fr_t*)jobs[i].C, (const fr_t*)jobs[i].scale, total});
+        total += jobs[i].n;
+    }
+    return launch_tasks(ctx, tasks, total, k_abc_batch, "pointwise_abc");
+}
+
+int gather_batch(hzk_ctx* ctx, const GatherJob* jobs, size_t count) {
+    std::vector<GatherTask> tasks;
+    uint64_t total = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const GatherJob& j = jobs[i];
+        HZK_CHECK(ctx, j.src && j.dst && j.elem_bytes % 16 == 0 && j.elem_bytes > 0 && (!j.r_host || j.elem_bytes == 32),
+                  HZK_ERR_INVALID, "gather: bad argument");
+        if (!j.n) continue;
+        GatherTask t{};
+        t.src = (const uint4*)j.src;
+        t.idx = j.idx;
+        t.dst = (uint4*)j.dst;
+        t.q = (uint32_t)(j.elem_bytes / 16);
+        t.scaled = j.r_host != nullptr;
+        if (t.scaled) std::memcpy(t.r.l, j.r_host, 32);
+        t.begin = total;
+        total += t.scaled ? j.n : j.n * t.q;
+        tasks.push_back(t);
+    }
+    return launch_tasks(ctx, tasks, total, k_gather_batch, "gather_rows");
 }
 
 }  // namespace hzk
