@@ -585,6 +585,15 @@ def main():
         "roofline": roofline,
     }
     log(f"msm: {value:.1f} Mpoints/s")
+    # the MSM's last step (Horner over the window sums) runs on the host; what
+    # the same ~255 dependent doublings would cost on one device lane
+    d1, d2 = ctx.roofline_probe("dbl_chain_g1"), ctx.roofline_probe("dbl_chain_g2")
+    out["msm_window_combine"] = {
+        "where": "host, after one D2H copy of the window sums (OpenMP over instances and windows)",
+        "device_chain_doubling_us": {"g1": 1e6 / d1, "g2": 1e6 / d2},
+        "device_horner_255_doublings_ms": {"g1": 255e3 / d1, "g2": 255e3 / d2},
+        "source": "hzk_roofline_probe kinds 4/5: dependent doublings of the 29-bit accumulator, one wave",
+    }
     if args.g16_proofs > 0:
         out["groth16_plain"] = groth16_plain(ctx, args.g16_log_m, args.g16_proofs, args.g16_levels)
         log(f"groth16 plain: {out['groth16_plain']['proofs_per_s']:.1f} proofs/s")

@@ -298,7 +298,9 @@ size_t hzk_timing_report(hzk_ctx* ctx, char* buf, size_t cap);
 /* ---- roofline denominators (no counterpart in the reference: the measured
  * peaks that achieved rates are divided by). kind: 0 v_mad_u64_u32 per second,
  * 1 Fq Montgomery products per second, 2 32-bit VALU adds per second,
- * 3 HBM copy bytes per second (read + write). */
+ * 3 HBM copy bytes per second (read + write), 4 / 5 G1 / G2 point doublings
+ * per second along one lane's dependent chain on a single wave (the latency
+ * a device-side window combine of about 255 doublings would pay). */
 int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value);
 
 /* ---- MSM autotuning (no counterpart in the reference, which hard-wires

@@ -164,9 +164,10 @@ class Context:
         return out
 
     def roofline_probe(self, kind):
-        """kind: 'mad' (v_mad_u64_u32/s), 'fqmul' (Fq products/s), 'add' (VALU adds/s), 'copy' (HBM bytes/s)."""
+        """kind: 'mad' (v_mad_u64_u32/s), 'fqmul' (Fq products/s), 'add' (VALU adds/s), 'copy' (HBM bytes/s),
+        'dbl_chain_g1' / 'dbl_chain_g2' (dependent point doublings/s on one lane of a single wave)."""
         v = ctypes.c_double()
-        code = {"mad": 0, "fqmul": 1, "add": 2, "copy": 3}[kind]
+        code = {"mad": 0, "fqmul": 1, "add": 2, "copy": 3, "dbl_chain_g1": 4, "dbl_chain_g2": 5}[kind]
         self._check(load().hzk_roofline_probe(self.handle, code, ctypes.byref(v)))
         return v.value
 

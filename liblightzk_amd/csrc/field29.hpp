@@ -66,6 +66,22 @@ struct fq29 {
         r.l[8] = w[7] >> 8;   // bits 232..255
         return r;
     }
+    // the same from two 16-byte words (low, high) with no array in between:
+    // a small private array indexed in a loop can be placed in LDS by the
+    // compiler, which turns every base load into LDS traffic
+    static HZK_DEV fq29 unpack(const uint4& lo, const uint4& hi) {
+        fq29 r;
+        r.l[0] = lo.x & MASK;
+        r.l[1] = ((lo.x >> 29) | (lo.y << 3)) & MASK;
+        r.l[2] = ((lo.y >> 26) | (lo.z << 6)) & MASK;
+        r.l[3] = ((lo.z >> 23) | (lo.w << 9)) & MASK;
+        r.l[4] = ((lo.w >> 20) | (hi.x << 12)) & MASK;
+        r.l[5] = ((hi.x >> 17) | (hi.y << 15)) & MASK;
+        r.l[6] = ((hi.y >> 14) | (hi.z << 18)) & MASK;
+        r.l[7] = ((hi.z >> 11) | (hi.w << 21)) & MASK;
+        r.l[8] = hi.w >> 8;
+        return r;
+    }
     // normalized limbs of a value < 2^256 back to 8 x 32-bit limbs
     HZK_DEV void pack(uint32_t* w) const {
 #pragma unroll

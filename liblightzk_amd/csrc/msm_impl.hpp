@@ -88,8 +88,7 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev
         }
         for (uint32_t e = e0; e < e1; ++e) {
             const bool neg = idx >> 31;
-            const uint32_t xw[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
-            const uint32_t yw[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
+            const uint4 x0 = p0, x1 = p1, y0 = p2, y1 = p3;
             const bool inf = p1.w == 0xffffffffu;   // the converter's mark for the point at infinity
             if (e + 1 < e1) {   // prefetch the next base while this one is added
                 const uint32_t t = (e + 1) & 3;
@@ -99,8 +98,8 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g1(MsmBatchDev
                 p0 = bp[0], p1 = bp[1], p2 = bp[2], p3 = bp[3];
             }
             if (!inf) {
-                const fq29 x = fq29::unpack(xw), y0 = fq29::unpack(yw);
-                const fq29 y = neg ? fq29::neg(y0, fq29::P1) : y0;
+                const fq29 x = fq29::unpack(x0, x1), ya = fq29::unpack(y0, y1);
+                const fq29 y = neg ? fq29::neg(ya, fq29::P1) : ya;
                 acc.add_affine(x, y);
             }
             if (e + 1 == next || e + 1 == e1) {
@@ -155,12 +154,8 @@ static __global__ void __launch_bounds__(256, 2) k_msm_accumulate_g2(MsmBatchDev
                 idx = t == 0 ? q.x : (t == 1 ? q.y : (t == 2 ? q.z : q.w));
             }
             if (p3.w != 0xffffffffu) {   // the converter's mark for the point at infinity
-                const uint32_t w0[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
-                const uint32_t w1[8] = {p2.x, p2.y, p2.z, p2.w, p3.x, p3.y, p3.z, p3.w};
-                const uint32_t w2[8] = {p4.x, p4.y, p4.z, p4.w, p5.x, p5.y, p5.z, p5.w};
-                const uint32_t w3[8] = {p6.x, p6.y, p6.z, p6.w, p7.x, p7.y, p7.z, p7.w};
-                const fq2_29 x{fq29::unpack(w0), fq29::unpack(w1)};
-                fq2_29 y{fq29::unpack(w2), fq29::unpack(w3)};
+                const fq2_29 x{fq29::unpack(p0, p1), fq29::unpack(p2, p3)};
+                fq2_29 y{fq29::unpack(p4, p5), fq29::unpack(p6, p7)};
                 if (neg) y = {fq29::neg(y.c0, fq29::P1), fq29::neg(y.c1, fq29::P1)};
                 acc.add_affine(x, y);
             }

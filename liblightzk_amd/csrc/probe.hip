@@ -1,9 +1,13 @@
 // Roofline denominators measured on the device itself: the issue rate of
 // v_mad_u64_u32 (the 32x32->64 integer multiply every field product is built
 // from), the rate of full 256-bit Montgomery products, plain 32-bit VALU adds,
-// and HBM copy bandwidth. bench.py divides achieved work by these.
+// and HBM copy bandwidth. bench.py divides achieved work by these. Kinds 4
+// and 5 time a dependent chain of point doublings (G1, G2; the MSM's 29-bit
+// accumulator formulas) on a single wave: the latency that a device-side
+// Horner combine of the window sums (about 255 sequential doublings) pays.
 #include "common.hpp"
 #include "field.hpp"
+#include "field29.hpp"
 
 namespace hzk {
 
@@ -52,6 +56,22 @@ __global__ void __launch_bounds__(256) k_probe_fqmul(fq_t* out, uint32_t iters) 
     out[t] = x + z;
 }
 
+// one wave: a chain of n doublings per lane from a lane-dependent start
+// (limbs below 2^29, the top one below 2^20: a value below p)
+template <class A>
+__global__ void __launch_bounds__(64) k_probe_dbl(uint4* out, uint32_t n) {
+    uint32_t w[A::RAW_WORDS];
+    uint32_t h = 0x9e3779b9u * (threadIdx.x + 1);
+    for (int i = 0; i < A::RAW_WORDS; ++i) {
+        h ^= h << 13, h ^= h >> 17, h ^= h << 5;
+        w[i] = (i % 9 == 8) ? (h & 0xfffffu) : (h & 0x1fffffffu);
+    }
+    A a = A::from_words(w);
+    a.inf = false;
+    for (uint32_t i = 0; i < n; ++i) a.dbl();
+    a.store_raw(out + (A::RAW_WORDS / 4) * threadIdx.x);
+}
+
 __global__ void __launch_bounds__(256) k_probe_copy(const uint4* __restrict__ in, uint4* __restrict__ out, size_t n) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
@@ -63,7 +83,7 @@ using namespace hzk;
 
 extern "C" int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value) {
     if (!ctx || !value) return HZK_ERR_INVALID;
-    HZK_CHECK(ctx, kind >= 0 && kind <= 3, HZK_ERR_INVALID, "probe: kind %d", kind);
+    HZK_CHECK(ctx, kind >= 0 && kind <= 5, HZK_ERR_INVALID, "probe: kind %d", kind);
     hipDeviceProp_t prop;
     HZK_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
     const uint32_t blocks = prop.multiProcessorCount * 32;   // 32 waves of 64 lanes... per CU: 8 per SIMD
@@ -91,6 +111,11 @@ extern "C" int hzk_roofline_probe(hzk_ctx* ctx, int kind, double* value) {
             const uint32_t iters = 8192;
             hipLaunchKernelGGL(k_probe_add, dim3(blocks), dim3(threads), 0, ctx->stream, (uint32_t*)ctx->scratch, iters, 3u);
             work = (double)lanes * iters * 8;
+        } else if (kind >= 4) {
+            const uint32_t n = 256;
+            if (kind == 4) hipLaunchKernelGGL(k_probe_dbl<g1acc29>, dim3(1), dim3(64), 0, ctx->stream, (uint4*)ctx->scratch, n);
+            else hipLaunchKernelGGL(k_probe_dbl<g2acc29>, dim3(1), dim3(64), 0, ctx->stream, (uint4*)ctx->scratch, n);
+            work = n;   // doublings along one lane's chain
         } else {
             size_t n = copy_bytes / 16;
             hipLaunchKernelGGL(k_probe_copy, dim3(prop.multiProcessorCount * 16), dim3(threads), 0, ctx->stream,
