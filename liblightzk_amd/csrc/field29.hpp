@@ -22,6 +22,22 @@
 
 namespace hzk {
 
+// acc += a * b as one v_mad_u64_u32 whose addend is the running column sum.
+// As plain C++ the compiler re-associates each column into partial chains
+// joined by extra 64-bit adds (17 per product); pinned this way those adds
+// disappear, but the compiler also separates consecutive inline multiplies
+// with s_nop, which only waves that share their SIMD with other waves hide:
+// the G1 accumulation (three waves per SIMD) uses it, the latency-bound
+// stages do not.
+HZK_DEV void mad_pinned(uint64_t& acc, uint32_t a, uint32_t b) {
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
+}
+template <bool PIN>
+HZK_DEV void mad29(uint64_t& acc, uint32_t a, uint32_t b) {
+    if constexpr (PIN) mad_pinned(acc, a, b);
+    else acc += (uint64_t)a * b;
+}
+
 struct fq29 {
     static constexpr uint32_t MASK = (1u << 29) - 1;
     static constexpr uint32_t P[9] = {0x187cfd47, 0x010460b6, 0x1c72a34f, 0x02d522d0, 0x1585d978,
@@ -157,7 +173,7 @@ struct fq29 {
     // N independent products in one interleaved scan: each multiply-add of
     // product c depends only on product c's accumulator, so interleaving
     // hides the multiply-add latency that a single product's chain exposes
-    template <int N>
+    template <int N, bool PIN = false>
     static HZK_DEV void mont_mul_n(fq29* r, const fq29* a, const fq29* b) {
         uint32_t m[N][9];
         uint64_t acc[N];
@@ -168,15 +184,15 @@ struct fq29 {
 #pragma unroll
             for (int j = 0; j < k; ++j)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
 #pragma unroll
             for (int j = 0; j <= k; ++j)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[j] * b[c].l[k - j];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[j], b[c].l[k - j]);
 #pragma unroll
             for (int c = 0; c < N; ++c) {
                 m[c][k] = ((uint32_t)acc[c] * M0) & MASK;
-                acc[c] += (uint64_t)m[c][k] * P[0];
+                mad29<PIN>(acc[c], m[c][k], P[0]);
                 acc[c] >>= 29;
             }
         }
@@ -185,11 +201,11 @@ struct fq29 {
 #pragma unroll
             for (int j = k - 8; j < 9; ++j)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[j] * b[c].l[k - j];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[j], b[c].l[k - j]);
 #pragma unroll
             for (int j = k - 8; j < 9; ++j)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
 #pragma unroll
             for (int c = 0; c < N; ++c) {
                 r[c].l[k - 9] = (uint32_t)acc[c] & MASK;
@@ -204,7 +220,7 @@ struct fq29 {
     // same integer, so the same result) with the doubled limbs precomputed:
     // 45 limb products instead of 81. Inputs normalized (limbs below 2^29,
     // doubled below 2^30; a column stays below 2^62.2).
-    template <int N>
+    template <int N, bool PIN = false>
     static HZK_DEV void mont_sqr_n(fq29* r, const fq29* a) {
         uint32_t m[N][9], d[N][9];
         uint64_t acc[N];
@@ -220,27 +236,27 @@ struct fq29 {
 #pragma unroll
                 for (int j = 0; j < k; ++j)
 #pragma unroll
-                    for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+                    for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
             }
 #pragma unroll
             for (int j = (k > 8 ? k - 8 : 0); 2 * j < k; ++j)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)d[c][j] * a[c].l[k - j];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], d[c][j], a[c].l[k - j]);
             if ((k & 1) == 0)
 #pragma unroll
-                for (int c = 0; c < N; ++c) acc[c] += (uint64_t)a[c].l[k / 2] * a[c].l[k / 2];
+                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[k / 2], a[c].l[k / 2]);
             if (k < 9) {
 #pragma unroll
                 for (int c = 0; c < N; ++c) {
                     m[c][k] = ((uint32_t)acc[c] * M0) & MASK;
-                    acc[c] += (uint64_t)m[c][k] * P[0];
+                    mad29<PIN>(acc[c], m[c][k], P[0]);
                     acc[c] >>= 29;
                 }
             } else {
 #pragma unroll
                 for (int j = k - 8; j < 9; ++j)
 #pragma unroll
-                    for (int c = 0; c < N; ++c) acc[c] += (uint64_t)m[c][j] * P[k - j];
+                    for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
 #pragma unroll
                 for (int c = 0; c < N; ++c) {
                     r[c].l[k - 9] = (uint32_t)acc[c] & MASK;
@@ -251,10 +267,11 @@ struct fq29 {
 #pragma unroll
         for (int c = 0; c < N; ++c) r[c].l[8] = (uint32_t)acc[c];
     }
+    template <bool PIN = false>
     static HZK_DEV void sqr2(fq29& r0, const fq29& a0, fq29& r1, const fq29& a1) {
         const fq29 a[2] = {a0, a1};
         fq29 r[2];
-        mont_sqr_n<2>(r, a);
+        mont_sqr_n<2, PIN>(r, a);
         r0 = r[0], r1 = r[1];
     }
     static HZK_DEV fq29 sqr1(const fq29& a0) {
@@ -262,17 +279,19 @@ struct fq29 {
         mont_sqr_n<1>(&r, &a0);
         return r;
     }
+    template <bool PIN = false>
     static HZK_DEV void mul2(fq29& r0, const fq29& a0, const fq29& b0, fq29& r1, const fq29& a1, const fq29& b1) {
         const fq29 a[2] = {a0, a1}, b[2] = {b0, b1};
         fq29 r[2];
-        mont_mul_n<2>(r, a, b);
+        mont_mul_n<2, PIN>(r, a, b);
         r0 = r[0], r1 = r[1];
     }
+    template <bool PIN = false>
     static HZK_DEV void mul3(fq29& r0, const fq29& a0, const fq29& b0, fq29& r1, const fq29& a1, const fq29& b1, fq29& r2,
                              const fq29& a2, const fq29& b2) {
         const fq29 a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2};
         fq29 r[3];
-        mont_mul_n<3>(r, a, b);
+        mont_mul_n<3, PIN>(r, a, b);
         r0 = r[0], r1 = r[1], r2 = r[2];
     }
     friend HZK_DEV fq29 operator*(const fq29& a, const fq29& b) { fq29 r; mont_mul(r, a, b); return r; }
@@ -371,7 +390,7 @@ struct g1acc29 {
     HZK_DEV void add_affine(const fq29& x, const fq29& y) {
         if (inf) { set_affine(x, y); return; }
         fq29 U2, S2;
-        fq29::mul2(U2, x, ZZ, S2, y, ZZZ);               // 1 x 2 each
+        fq29::mul2<true>(U2, x, ZZ, S2, y, ZZZ);               // 1 x 2 each
         fq29 P = fq29::sub(U2, X, fq29::P8);             // < 10
         fq29 R = fq29::sub(S2, Y, fq29::P4);             // < 6
         bool dbl = false;
@@ -383,15 +402,15 @@ struct g1acc29 {
             dbl = true;
         }
         fq29 PP, RR;
-        fq29::sqr2(PP, P, RR, R);                  // 10 x 10, 6 x 6
+        fq29::sqr2<true>(PP, P, RR, R);                  // 10 x 10, 6 x 6
         fq29 PPP, Q, ZZ3;
-        fq29::mul3(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);   // 10 x 2, 8 x 2, 2 x 2
+        fq29::mul3<true>(PPP, P, PP, Q, X, PP, ZZ3, ZZ, PP);   // 10 x 2, 8 x 2, 2 x 2
         fq29 T = Q + Q;
         if (!dbl) T = T + PPP;
         X = fq29::sub(RR, T, fq29::P6);                  // RR - (PPP + 2Q) + 6p < 8
         const fq29 D = fq29::sub(Q, X, fq29::P8);        // < 10
         fq29 t, YP, ZZZ3;
-        fq29::mul3(t, R, D, YP, Y, PPP, ZZZ3, ZZZ, PPP); // 6 x 10, 4 x 2, 2 x 2
+        fq29::mul3<true>(t, R, D, YP, Y, PPP, ZZZ3, ZZZ, PPP); // 6 x 10, 4 x 2, 2 x 2
         Y = fq29::sub(t, YP, fq29::P2);                  // < 4
         ZZ = ZZ3, ZZZ = ZZZ3;
     }
