@@ -38,6 +38,14 @@ typedef struct hzk_ctx hzk_ctx;
 /* batch flags */
 #define HZK_BATCH_HOST_DEST 1u   /* MSM results written to host pointers */
 #define HZK_BATCH_GRAPH 2u       /* capture the device stages into a hipGraph and replay it */
+/* the window combine (part sums, Horner over the windows, normalisation) as a
+ * device kernel, one lane per instance, so the call returns without waiting
+ * for the device and, with HZK_BATCH_GRAPH, the whole MSM is one graph;
+ * dest must be device memory. It ends the MSM later than the host combine
+ * (about 255 dependent doublings on one lane: 1 ms G1, 3 ms G2 on MI355X,
+ * hzk_roofline_probe kinds 4/5); the reference also ends its MSM on the
+ * device (msm.cuh:226-227). hzk_msm_last_stats is not updated in this mode. */
+#define HZK_BATCH_DEVICE_COMBINE 4u
 
 /* ---- context (replaces the reference's implicit CUDA runtime state and legacy
  * default stream, include/device-arith/device-utils.cuh:3) ---- */

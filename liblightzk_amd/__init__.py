@@ -220,13 +220,14 @@ class Context:
     def coset_ntt(self, coeffs, deg, inverse=False):
         self._check(load().hzk_coset_ntt(self.handle, int(coeffs), deg, 1 if inverse else 0))
 
-    def msm_batch(self, group, jobs, host_dest=False, graph=False):
+    def msm_batch(self, group, jobs, host_dest=False, graph=False, device_combine=False):
         """jobs: list of (scalars, bases, n, dest[, win_siz[, levels]]); device pointers (dest host if
-        host_dest); levels != 0 marks bases as a level table built by bases_levels for win_siz."""
+        host_dest); levels != 0 marks bases as a level table built by bases_levels for win_siz.
+        device_combine: the window combine runs on the device and the call returns without waiting."""
         arr = (MsmDesc * len(jobs))()
         for i, j in enumerate(jobs):
             arr[i] = MsmDesc(int(j[0]), int(j[1]), j[2], int(j[3]), j[4] if len(j) > 4 else 0, j[5] if len(j) > 5 else 0)
-        flags = (1 if host_dest else 0) | (2 if graph else 0)
+        flags = (1 if host_dest else 0) | (2 if graph else 0) | (4 if device_combine else 0)
         fn = load().hzk_msm_g1_batch if group == 1 else load().hzk_msm_g2_batch
         self._check(fn(self.handle, arr, len(jobs), flags))
 

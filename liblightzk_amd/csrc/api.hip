@@ -378,11 +378,15 @@ int hzk_msm_g2(hzk_ctx* ctx, const void* scalars, const void* bases, size_t n, u
 static int msm_batch_entry(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags, bool g2) {
     if (!ctx) return HZK_ERR_INVALID;
     HZK_CHECK(ctx, d && count > 0, HZK_ERR_INVALID, "msm batch: empty");
-    HZK_CHECK(ctx, (flags & ~(HZK_BATCH_HOST_DEST | HZK_BATCH_GRAPH)) == 0, HZK_ERR_INVALID, "msm batch: bad flags %u", flags);
+    HZK_CHECK(ctx, (flags & ~(HZK_BATCH_HOST_DEST | HZK_BATCH_GRAPH | HZK_BATCH_DEVICE_COMBINE)) == 0, HZK_ERR_INVALID,
+              "msm batch: bad flags %u", flags);
+    HZK_CHECK(ctx, !((flags & HZK_BATCH_HOST_DEST) && (flags & HZK_BATCH_DEVICE_COMBINE)), HZK_ERR_INVALID,
+              "msm batch: the device-side combine writes device results (no HZK_BATCH_HOST_DEST)");
     std::vector<MsmJob> jobs(count);
     for (size_t i = 0; i < count; ++i) jobs[i] = {d[i].scalars, d[i].bases, d[i].n, d[i].dest, d[i].win_siz, d[i].levels};
     int host = (flags & HZK_BATCH_HOST_DEST) ? 1 : 0, graph = (flags & HZK_BATCH_GRAPH) ? 1 : 0;
-    return g2 ? msm_batch_g2(ctx, jobs.data(), count, host, graph) : msm_batch_g1(ctx, jobs.data(), count, host, graph);
+    int dev = (flags & HZK_BATCH_DEVICE_COMBINE) ? 1 : 0;
+    return g2 ? msm_batch_g2(ctx, jobs.data(), count, host, graph, dev) : msm_batch_g1(ctx, jobs.data(), count, host, graph, dev);
 }
 int hzk_msm_g1_batch(hzk_ctx* ctx, const hzk_msm_desc* d, size_t count, uint32_t flags) {
     return msm_batch_entry(ctx, d, count, flags, false);

@@ -79,6 +79,7 @@ struct MsmInst {
     // of a few bits whose digits pile into a handful of buckets. Level tables
     // keep uniform windows of c bits (wb = c, wx = 0).
     uint32_t wb, wx;
+    void* dest;            // the result (device), for the device-side combine
 };
 
 struct MsmBatchDev {
@@ -135,13 +136,15 @@ struct MsmJob {
     uint32_t levels;   // 0 or 1: affine bases; else levels per bucket set of a level table (<= ceil(255 / win_siz))
     uint32_t form = 0; // 0 affine bases, 1 bases already converted by msm_convert_g1 / _g2 (kept by the prover)
 };
-int msm_batch_g1(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
+// device_combine: the window combine runs as a kernel into device dests and
+// the call returns without waiting (HZK_BATCH_DEVICE_COMBINE)
+int msm_batch_g1(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture, int device_combine = 0);
 // G1 affine bases -> the form the G1 bucket accumulation reads (field29.hpp's
 // 2^261 Montgomery coordinates, infinity marked), n points, on ctx's stream;
 // dst may equal src
 int msm_convert_g1(hzk_ctx* ctx, const void* src, void* dst, size_t n);
 // the same for G2 affine bases (128 bytes each)
 int msm_convert_g2(hzk_ctx* ctx, const void* src, void* dst, size_t n);
-int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture);
+int msm_batch_g2(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture, int device_combine = 0);
 
 }  // namespace hzk

@@ -567,6 +567,111 @@ __global__ void __launch_bounds__(T) k_window_sum(MsmBatchDev b, const uint4* __
     }
 }
 
+// ------------------------------------------------------------------ 8. device-side combine (optional)
+// The host's combine as two kernels, in the 29-bit accumulator arithmetic.
+// k_msm_combine_windows: workgroup per (instance, window), lane per part p:
+// T_j = sum_p P_p + (g Gs) sum_p p RS_p, the last sum as the sum over p >= 1
+// of the suffix sums of RS (an LDS scan), stored as a raw state in the
+// run-sum area (free once the window sums exist). k_msm_combine_horner: lane
+// per instance, Horner over the windows (the dependent doublings that make
+// this path slower than the host's), then the normalised Jacobian
+// (x, y, 1) or (0, 1, 0): the bytes the host writes.
+template <class F>
+HZK_DEV acc29_t<F> acc_from_xyzz(const xyzz_t<F>& p);
+template <>
+HZK_DEV g1acc29 acc_from_xyzz<fq_t>(const xyzz_t<fq_t>& p) {
+    g1acc29 a;
+    a.inf = p.ZZ.is_zero();
+    fq29::mul2(a.X, fq29::unpack(p.X.l), fq29::konst(fq29::TO), a.Y, fq29::unpack(p.Y.l), fq29::konst(fq29::TO));
+    fq29::mul2(a.ZZ, fq29::unpack(p.ZZ.l), fq29::konst(fq29::TO), a.ZZZ, fq29::unpack(p.ZZZ.l), fq29::konst(fq29::TO));
+    return a;
+}
+template <>
+HZK_DEV g2acc29 acc_from_xyzz<fq2_t>(const xyzz_t<fq2_t>& p) {
+    g2acc29 a;
+    a.inf = p.ZZ.is_zero();
+    const fq29 T = fq29::konst(fq29::TO);
+    fq29::mul2(a.X.c0, fq29::unpack(p.X.c0.l), T, a.X.c1, fq29::unpack(p.X.c1.l), T);
+    fq29::mul2(a.Y.c0, fq29::unpack(p.Y.c0.l), T, a.Y.c1, fq29::unpack(p.Y.c1.l), T);
+    fq29::mul2(a.ZZ.c0, fq29::unpack(p.ZZ.c0.l), T, a.ZZ.c1, fq29::unpack(p.ZZ.c1.l), T);
+    fq29::mul2(a.ZZZ.c0, fq29::unpack(p.ZZZ.c0.l), T, a.ZZZ.c1, fq29::unpack(p.ZZZ.c1.l), T);
+    return a;
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_combine_windows(MsmBatchDev b, const xyzz_t<F>* __restrict__ win, uint32_t maxw,
+                                                            uint4* __restrict__ tsum) {
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
+    __shared__ uint32_t shw[LdsPoints<F, 64>::W * 64];
+    const LdsPoints<F, 64> sh{shw};
+    const uint32_t i = blockIdx.x / maxw, j = blockIdx.x % maxw, t = threadIdx.x;
+    const MsmInst& d = b.inst[i];
+    if (j >= d.W) return;   // uniform per workgroup
+    const xyzz_t<F>* pj = win + 2 * (space_g(b, SP_WIN)[i] + (size_t)j * d.S);
+    A P, RS;
+    P.set_inf();
+    RS.set_inf();
+    if (t < d.S) {
+        P = acc_from_xyzz<F>(pj[2 * t]);
+        RS = acc_from_xyzz<F>(pj[2 * t + 1]);
+    }
+    // inclusive suffix scan of RS: slot t = sum_{p >= t} RS_p
+    sh.put(t, RS);
+    __syncthreads();
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        A x = sh.get(t);
+        if (t + off < 64) x.add(sh.get(t + off));
+        __syncthreads();
+        sh.put(t, x);
+        __syncthreads();
+    }
+    A y = sh.get(t);
+    if (t == 0) y.set_inf();   // sum_p p RS_p = sum_{t >= 1} slot t
+    __syncthreads();
+    sh.put(t, y);
+    __syncthreads();
+    lds_tree_sum<F, 64>(sh, t);
+    A ysum = sh.get(0);
+    __syncthreads();
+    sh.put(t, P);
+    __syncthreads();
+    lds_tree_sum<F, 64>(sh, t);
+    if (t == 0) {
+        A T = sh.get(0);
+        for (uint32_t k = 1; k < d.G / d.S * d.g; k <<= 1) ysum.dbl();
+        T.add(ysum);
+        T.store_raw(tsum + (size_t)RW * (space_g(b, SP_BRED)[i] + j));
+    }
+}
+
+template <class F>
+__global__ void __launch_bounds__(64) k_msm_combine_horner(MsmBatchDev b, const uint4* __restrict__ tsum) {
+    using A = acc29_t<F>;
+    constexpr int RW = raw_vec<F>();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.ninst) return;
+    const MsmInst& d = b.inst[i];
+    const uint4* T = tsum + (size_t)RW * space_g(b, SP_BRED)[i];
+    A acc;
+    acc.set_inf();
+    for (int j = (int)d.W - 1; j >= 0; --j) {
+        const uint32_t shift = d.Lg > 1 ? d.c * d.Lg : d.wb + ((uint32_t)j < d.wx ? 1u : 0u);
+        for (uint32_t q = 0; q < shift; ++q) acc.dbl();
+        acc.add(A::load_raw(T + (size_t)RW * j));
+    }
+    const xyzz_t<F> r = acc.to_xyzz();
+    F* out = (F*)d.dest;
+    if (r.is_inf()) {
+        out[0] = F::zero(), out[1] = F::one(), out[2] = F::zero();
+        return;
+    }
+    const F inv = (r.ZZ * r.ZZZ).inverse();   // 1 / (ZZ ZZZ)
+    out[0] = r.X * (r.ZZZ * inv);             // X / ZZ
+    out[1] = r.Y * (r.ZZ * inv);              // Y / ZZZ
+    out[2] = F::one();
+}
+
 // ------------------------------------------------------------------ driver
 struct MsmBatchLayout {
     std::vector<MsmInst> inst;
@@ -612,6 +717,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         MsmInst& d = L.inst[i];
         d.scalars = jobs[i].scalars;
         d.bases = jobs[i].bases;
+        d.dest = jobs[i].dest;
         d.n = (uint32_t)jobs[i].n;
         d.c = (uint32_t)p.c;
         d.W = (uint32_t)p.W;
@@ -652,7 +758,7 @@ static MsmBatchLayout msm_layout(hzk_ctx* ctx, const MsmJob* jobs, size_t count)
         // of 2g in a run-sum pass and then up to 28), while the W * G / 256
         // workgroups of four waves fit one wave per SIMD (a second round of
         // waves would double the kernel's latency-bound time)
-        d.fused = regime == MSM_REGIME_SINGLE && d.G % 256 == 0 && (uint64_t)d.W * (d.G / 256) <= 256;
+        d.fused = regime == MSM_REGIME_SINGLE && d.G % 256 == 0 && d.G / 256 <= 64 && (uint64_t)d.W * (d.G / 256) <= 256;
         if (d.fused) d.S = d.G / 256;
         uint64_t entries = (uint64_t)d.n * d.W;
         uint64_t sz[SP_COUNT];
@@ -803,9 +909,27 @@ static void msm_device_stages(hzk_ctx* ctx, const MsmBatchLayout& L, char* base)
 }
 
 template <class F>
-static std::string shape_key(const MsmBatchLayout& L) {
+static void msm_launch_combine(hzk_ctx* ctx, const MsmBatchLayout& L, char* base) {
+    MsmBatchDev b;
+    b.inst = (const MsmInst*)(base + L.o_inst);
+    b.begin = (const uint64_t*)(base + L.o_begin);
+    b.ninst = (uint32_t)L.inst.size();
+    b.stats = (uint32_t*)(base + L.o_stats);
+    uint32_t maxw = 1;
+    for (const MsmInst& d : L.inst) maxw = std::max(maxw, d.W);
+    uint4* tsum = (uint4*)(base + L.o_segsum);   // W raw states per instance fit its run-sum range
+    TimedLaunch tl(ctx, "msm_combine");
+    hipLaunchKernelGGL(k_msm_combine_windows<F>, dim3((unsigned)(L.inst.size() * maxw)), dim3(64), 0, ctx->stream, b,
+                       (const xyzz_t<F>*)(base + L.o_win), maxw, tsum);
+    hipLaunchKernelGGL(k_msm_combine_horner<F>, dim3(blocks_for(L.inst.size(), 64)), dim3(64), 0, ctx->stream, b,
+                       (const uint4*)tsum);
+}
+
+template <class F>
+static std::string shape_key(const MsmBatchLayout& L, bool combine) {
     std::string k(sizeof(F) == 32 ? "g1:" : "g2:");
     k += L.regime == MSM_REGIME_BATCH ? "b:" : "s:";
+    if (combine) k += "dev:";
     k += std::to_string(L.conv.size()) + "/" + std::to_string(L.conv_total) + ":";
     for (const MsmInst& d : L.inst) {
         char buf[64];
@@ -820,7 +944,7 @@ static std::string shape_key(const MsmBatchLayout& L) {
 // the device or pinned on the host (no graph capture in this mode)
 template <class F, class HostPt>
 int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_host, int capture, void* ext_scratch = nullptr,
-                  size_t ext_bytes = 0) {
+                  size_t ext_bytes = 0, int device_combine = 0) {
     HZK_CHECK(ctx, count > 0 && jobs, HZK_ERR_INVALID, "msm: empty batch");
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].n > 0 && jobs[i].n < (1ull << 31), HZK_ERR_INVALID, "msm: n=%zu out of range [1, 2^31)", jobs[i].n);
@@ -833,6 +957,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HZK_CHECK(ctx, jobs[i].win_siz == 0 || (jobs[i].win_siz >= (uint32_t)MSM_MIN_C && jobs[i].win_siz <= (uint32_t)MSM_MAX_C),
                   HZK_ERR_INVALID, "msm: win_siz=%u unsupported (0 or %d..%d)", jobs[i].win_siz, MSM_MIN_C, MSM_MAX_C);
     }
+    HZK_CHECK(ctx, !(device_combine && dests_on_host), HZK_ERR_INVALID, "msm: the device-side combine needs device dests");
     msm_prepare();
     MsmBatchLayout L = msm_layout<F>(ctx, jobs, count);
     for (size_t i = 0; i < count; ++i)   // bucket offsets are 32-bit
@@ -883,7 +1008,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     }
 
     if (capture) {
-        std::string key = shape_key<F>(L);
+        std::string key = shape_key<F>(L, device_combine != 0);
         hzk_graph_entry& g = ctx->graphs[key];
         if (!g.exec || g.generation != ctx->scratch_generation) {
             if (g.exec) { (void)hipGraphExecDestroy(g.exec); g.exec = nullptr; }
@@ -892,6 +1017,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
             ctx->timing = false;
             HZK_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             msm_device_stages<F>(ctx, L, base);
+            if (device_combine) msm_launch_combine<F>(ctx, L, base);
             hipError_t e = hipStreamEndCapture(st, &graph);
             ctx->timing = timing;
             HZK_HIP(ctx, e);
@@ -904,8 +1030,14 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
         HZK_HIP(ctx, hipGraphLaunch(g.exec, st));
     } else {
         msm_device_stages<F>(ctx, L, base);
+        if (device_combine) msm_launch_combine<F>(ctx, L, base);
     }
     HZK_HIP(ctx, hipGetLastError());
+    if (device_combine) {   // results are written by the device; nothing comes back to the host
+        if (slot >= 0) return stage_fence(ctx, slot);
+        HZK_HIP(ctx, hipStreamSynchronize(st));   // pageable staging (no-alloc scratch): wait for the copies
+        return 0;
+    }
     char* hwin = pin + count * sizeof(MsmInst) + L.begin.size() * 8;
     uint32_t* hstats = (uint32_t*)(hwin + 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>));
     HZK_HIP(ctx, hipMemcpyAsync(hwin, base + L.o_win, 2 * L.tot[SP_WIN] * sizeof(xyzz_t<F>), hipMemcpyDeviceToHost, st));

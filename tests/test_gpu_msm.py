@@ -200,3 +200,50 @@ def test_g2_batch_host_dest(ctx, orc):
         want = orc.msm_closed_form(s[4 * off: 4 * (off + n)].copy(), k[4 * off: 4 * (off + n)].copy(), 2)
         assert np.array_equal(host_out[i], want), (i, n)
         off += n
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_device_combine_matches_host_combine(ctx, orc, graph):
+    """HZK_BATCH_DEVICE_COMBINE (the window combine as a kernel, no host
+    synchronisation inside the call): the same bytes as the host combine and
+    the closed form, G1 and G2, including an all-zero instance (the point at
+    infinity) and a replayed graph."""
+    rng = np.random.default_rng(11)
+    for group in (1, 2):
+        sizes = [int(2 ** rng.uniform(3, 12)) for _ in range(10)] + [1, 37]
+        s, k, bases = _batch_case(orc, 31 + group, sizes, group)
+        s[: 4 * sizes[0]] = 0
+        pw, bw = (96, 64) if group == 1 else (192, 128)
+        ds, db = ctx.upload(s), ctx.upload(bases)
+        out_h, out_d = ctx.alloc(pw * len(sizes)), ctx.alloc(pw * len(sizes))
+
+        def jobs(out):
+            js, off = [], 0
+            for i, n in enumerate(sizes):
+                js.append((ds.offset(32 * off), db.offset(bw * off), n, out.offset(pw * i)))
+                off += n
+            return js
+
+        ctx.msm_batch(group, jobs(out_h), graph=graph)
+        want = out_h.download(np.uint64, pw // 8 * len(sizes))
+        for rep in range(2):
+            out_d.zero()
+            ctx.msm_batch(group, jobs(out_d), graph=graph, device_combine=True)
+            ctx.sync()
+            got = out_d.download(np.uint64, pw // 8 * len(sizes))
+            assert np.array_equal(got, want), (group, rep)
+        off = 0
+        for i, n in enumerate(sizes):
+            cf = orc.msm_closed_form(s[4 * off: 4 * (off + n)].copy(), k[4 * off: 4 * (off + n)].copy(), group)
+            assert np.array_equal(want[pw // 8 * i: pw // 8 * (i + 1)], cf), (group, i, n)
+            off += n
+        for b in (ds, db, out_h, out_d):
+            b.free()
+
+
+def test_device_combine_rejects_host_dest(ctx):
+    import liblightzk_amd as z
+    d = ctx.alloc(64)
+    with pytest.raises(z.HzkError):
+        ctx.msm_batch(1, [(d.ptr, d.ptr, 1, 0)], host_dest=True, device_combine=True)
+    d.free()
