@@ -25,10 +25,11 @@ namespace hzk {
 // acc += a * b as one v_mad_u64_u32 whose addend is the running column sum.
 // As plain C++ the compiler re-associates each column into partial chains
 // joined by extra 64-bit adds (17 per product); pinned this way those adds
-// disappear, but the compiler also separates consecutive inline multiplies
-// with s_nop, which only waves that share their SIMD with other waves hide:
-// the G1 accumulation (three waves per SIMD) uses it, the latency-bound
-// stages do not.
+// disappear, but the compiler also puts an s_nop between consecutive inline
+// asm blocks (mad_row groups the interleaved products' multiplies of one
+// column step into one block to have fewer of them), which only waves that
+// share their SIMD with other waves hide: the accumulations (two or three
+// waves per SIMD) use it, the latency-bound stages do not.
 HZK_DEV void mad_pinned(uint64_t& acc, uint32_t a, uint32_t b) {
     asm("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
 }
@@ -36,6 +37,23 @@ template <bool PIN>
 HZK_DEV void mad29(uint64_t& acc, uint32_t a, uint32_t b) {
     if constexpr (PIN) mad_pinned(acc, a, b);
     else acc += (uint64_t)a * b;
+}
+// acc[c] += x[c] * y[c] for the N interleaved products of one column step,
+// pinned as one asm block (independent multiplies, no s_nop between them)
+template <int N, bool PIN>
+HZK_DEV void mad_row(uint64_t* acc, const uint32_t* x, const uint32_t* y) {
+    if constexpr (!PIN || N > 3) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], x[c], y[c]);
+    } else if constexpr (N == 1) {
+        asm("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc[0]) : "v"(x[0]), "v"(y[0]) : "vcc");
+    } else if constexpr (N == 2) {
+        asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_mad_u64_u32 %1, vcc, %4, %5, %1"
+            : "+v"(acc[0]), "+v"(acc[1]) : "v"(x[0]), "v"(y[0]), "v"(x[1]), "v"(y[1]) : "vcc");
+    } else {
+        asm("v_mad_u64_u32 %0, vcc, %3, %4, %0\n\tv_mad_u64_u32 %1, vcc, %5, %6, %1\n\tv_mad_u64_u32 %2, vcc, %7, %8, %2"
+            : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]) : "v"(x[0]), "v"(y[0]), "v"(x[1]), "v"(y[1]), "v"(x[2]), "v"(y[2]) : "vcc");
+    }
 }
 
 struct fq29 {
@@ -183,12 +201,20 @@ struct fq29 {
         for (int k = 0; k < 9; ++k) {
 #pragma unroll
             for (int j = 0; j < k; ++j)
+                {
+                    uint32_t x[N], y[N];
 #pragma unroll
-                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
+                    for (int c = 0; c < N; ++c) x[c] = m[c][j], y[c] = P[k - j];
+                    mad_row<N, PIN>(acc, x, y);
+                }
 #pragma unroll
             for (int j = 0; j <= k; ++j)
+                {
+                    uint32_t x[N], y[N];
 #pragma unroll
-                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[j], b[c].l[k - j]);
+                    for (int c = 0; c < N; ++c) x[c] = a[c].l[j], y[c] = b[c].l[k - j];
+                    mad_row<N, PIN>(acc, x, y);
+                }
 #pragma unroll
             for (int c = 0; c < N; ++c) {
                 m[c][k] = ((uint32_t)acc[c] * M0) & MASK;
@@ -200,12 +226,20 @@ struct fq29 {
         for (int k = 9; k < 17; ++k) {
 #pragma unroll
             for (int j = k - 8; j < 9; ++j)
+                {
+                    uint32_t x[N], y[N];
 #pragma unroll
-                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[j], b[c].l[k - j]);
+                    for (int c = 0; c < N; ++c) x[c] = a[c].l[j], y[c] = b[c].l[k - j];
+                    mad_row<N, PIN>(acc, x, y);
+                }
 #pragma unroll
             for (int j = k - 8; j < 9; ++j)
+                {
+                    uint32_t x[N], y[N];
 #pragma unroll
-                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], m[c][j], P[k - j]);
+                    for (int c = 0; c < N; ++c) x[c] = m[c][j], y[c] = P[k - j];
+                    mad_row<N, PIN>(acc, x, y);
+                }
 #pragma unroll
             for (int c = 0; c < N; ++c) {
                 r[c].l[k - 9] = (uint32_t)acc[c] & MASK;
@@ -240,8 +274,12 @@ struct fq29 {
             }
 #pragma unroll
             for (int j = (k > 8 ? k - 8 : 0); 2 * j < k; ++j)
+                {
+                    uint32_t x[N], y[N];
 #pragma unroll
-                for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], d[c][j], a[c].l[k - j]);
+                    for (int c = 0; c < N; ++c) x[c] = d[c][j], y[c] = a[c].l[k - j];
+                    mad_row<N, PIN>(acc, x, y);
+                }
             if ((k & 1) == 0)
 #pragma unroll
                 for (int c = 0; c < N; ++c) mad29<PIN>(acc[c], a[c].l[k / 2], a[c].l[k / 2]);
@@ -525,14 +563,16 @@ struct fq2_29 {
     // the same products in one interleaved scan (three independent chains of
     // multiply-adds): for kernels at one wave per SIMD, where a single
     // product's dependent chain would leave the SIMD waiting
+    template <bool PIN = false>
     static HZK_DEV fq2_29 mul_x(const fq2_29& a, const fq2_29& b) {
         fq29 v0, v1, v2;
-        fq29::mul3(v0, a.c0, b.c0, v1, a.c1, b.c1, v2, a.c0 + a.c1, b.c0 + b.c1);
+        fq29::mul3<PIN>(v0, a.c0, b.c0, v1, a.c1, b.c1, v2, a.c0 + a.c1, b.c0 + b.c1);
         return {fq29::sub(v0, v1, fq29::P2), fq29::sub(v2, v0 + v1, fq29::P4)};
     }
+    template <bool PIN = false>
     HZK_DEV fq2_29 sqr_x() const {
         fq29 t, ab;
-        fq29::mul2(t, c0 + c1, fq29::sub(c0, c1, fq29::P2), ab, c0, c1);
+        fq29::mul2<PIN>(t, c0 + c1, fq29::sub(c0, c1, fq29::P2), ab, c0, c1);
         return {t, (ab + ab).norm()};
     }
     // (a0 + a1)(a0 - a1) and 2 a0 a1, 2 interleaved products; components below 2p:
@@ -562,26 +602,26 @@ struct g2acc29 {
     // registers)
     HZK_DEV void add_affine(const fq2_29& x, const fq2_29& y) {
         if (inf) { set_affine(x, y); return; }
-        fq2_29 P = fq2_29::sub(fq2_29::mul_x(x, ZZ), X, fq29::P2).reduce();                // U2 = x ZZ: (2)(10); P < 2
-        fq2_29 R = fq2_29::sub(fq2_29::mul_x(y, ZZZ), Y, fq29::P2).reduce();               // S2 = y ZZZ; R < 2
+        fq2_29 P = fq2_29::sub(fq2_29::mul_x<true>(x, ZZ), X, fq29::P2).reduce();                // U2 = x ZZ: (2)(10); P < 2
+        fq2_29 R = fq2_29::sub(fq2_29::mul_x<true>(y, ZZZ), Y, fq29::P2).reduce();               // S2 = y ZZZ; R < 2
         bool dbl = false;
         if (P.is_zero_mod_p()) {
             if (!R.is_zero_mod_p()) { set_inf(); return; }
             P = (Y + Y).norm().reduce();                                     // 2Y < 2
-            const fq2_29 xx = X.sqr_x();
+            const fq2_29 xx = X.template sqr_x<true>();
             R = (xx + xx + xx).norm().reduce();                              // 3X^2 < 2
             dbl = true;
         }
-        const fq2_29 PP = P.sqr_x();                                           // < 4
-        const fq2_29 PPP = fq2_29::mul_x(P, PP);                                           // (4)(6), < 6
-        ZZ = fq2_29::mul_x(ZZ, PP);                                                        // (10)(6)
-        const fq2_29 Q = fq2_29::mul_x(X, PP);                                             // (4)(6), < 6
+        const fq2_29 PP = P.template sqr_x<true>();                                           // < 4
+        const fq2_29 PPP = fq2_29::mul_x<true>(P, PP);                                           // (4)(6), < 6
+        ZZ = fq2_29::mul_x<true>(ZZ, PP);                                                        // (10)(6)
+        const fq2_29 Q = fq2_29::mul_x<true>(X, PP);                                             // (4)(6), < 6
         fq2_29 T = Q + Q;
         if (!dbl) T = T + PPP;
-        X = fq2_29::sub(R.sqr_x(), T, fq29::P18).reduce();                     // RR - (PPP + 2Q) + 18p, then < 2
-        const fq2_29 t = fq2_29::mul_x(R, fq2_29::sub(Q, X, fq29::P2).reduce());           // R (Q - X3): (4)(4)
-        Y = fq2_29::sub(t, fq2_29::mul_x(Y, PPP), fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
-        ZZZ = fq2_29::mul_x(ZZZ, PPP);                                                     // (10)(10)
+        X = fq2_29::sub(R.template sqr_x<true>(), T, fq29::P18).reduce();                     // RR - (PPP + 2Q) + 18p, then < 2
+        const fq2_29 t = fq2_29::mul_x<true>(R, fq2_29::sub(Q, X, fq29::P2).reduce());           // R (Q - X3): (4)(4)
+        Y = fq2_29::sub(t, fq2_29::mul_x<true>(Y, PPP), fq29::P6).reduce();                      // Y PPP: (4)(10); Y3 < 2
+        ZZZ = fq2_29::mul_x<true>(ZZZ, PPP);                                                     // (10)(10)
     }
     // raw state: 72 words, infinity has ZZ = 0 (read back by load_raw)
     static constexpr int RAW_WORDS = 72;
