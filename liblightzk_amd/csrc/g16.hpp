@@ -23,6 +23,10 @@ struct hzk_g16 {
     void *A1 = nullptr, *B1 = nullptr, *B2 = nullptr, *L1 = nullptr, *H1 = nullptr;
     hzk::host::G1x alpha1, beta1, delta1;
     hzk::host::G2x beta2, gamma2, delta2;
+    // byte-window tables of delta1, beta1, delta2 for the proofs' r delta1,
+    // r beta1 and s delta2 (built with the prover buffers)
+    hzk::host::FixedBase<hzk::host::Fq> fb_delta1, fb_beta1;
+    hzk::host::FixedBase<hzk::host::Fq2> fb_delta2;
     std::vector<hzk::host::G1x> IC;   // verifying-key input commitments
     // buffers of the plain one-proof-at-a-time prover, and 1/(g^M - 1)
     void *z = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *zinv = nullptr;

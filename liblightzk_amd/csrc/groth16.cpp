@@ -370,6 +370,9 @@ int hzk::g16_prover_buffers(hzk_ctx* ctx, hzk_g16* g) {
     if (!rc) rc = dev_alloc(ctx, g, g->vec_c * 32, &g->c);
     if (rc) return rc;
     Fr zi = (fr_pow_u64(fr_u64(5), M) - Fr::one()).inverse();
+    g->fb_delta1.build(g->delta1);
+    g->fb_beta1.build(g->beta1);
+    g->fb_delta2.build(g->delta2);
     return dev_upload(ctx, g, zi.v, 32, &g->zinv);
 }
 
@@ -497,7 +500,7 @@ G16Fixed g16_fixed_terms(const hzk_g16* g, const void* rs) {
     uint64_t rc[4], sc[4];
     r.canonical(rc);
     s.canonical(sc);
-    return {g->delta1.mul(rc), g->beta1.mul(rc), g->delta2.mul(sc)};
+    return {g->fb_delta1.mul(rc), g->fb_beta1.mul(rc), g->fb_delta2.mul(sc)};
 }
 
 // Phase 6 on the host; proof = A (64 B) | B (128 B) | C (64 B), affine.

@@ -187,6 +187,36 @@ struct Xyzz {
     }
 };
 
+// k * P for a point P fixed for many products (a proving key's delta and
+// beta): 32 byte windows, entry (w, b) = b 2^(8w) P for b = 1..255, so a
+// product is at most 32 additions and no doublings (double-and-add: 256
+// doublings and ~128 additions)
+template <class F>
+struct FixedBase {
+    std::vector<Xyzz<F>> t;   // 32 x 256 entries, b = 0 unused
+    void build(const Xyzz<F>& p) {
+        t.assign(32 * 256, Xyzz<F>::inf());
+        Xyzz<F> base = p;
+        for (int w = 0; w < 32; ++w) {
+            Xyzz<F> acc = Xyzz<F>::inf();
+            for (int b = 1; b < 256; ++b) {
+                acc = acc.add(base);
+                t[w * 256 + b] = acc;
+            }
+            for (int q = 0; q < 8; ++q) base = base.dbl();
+        }
+    }
+    bool ready() const { return !t.empty(); }
+    Xyzz<F> mul(const uint64_t k[4]) const {
+        Xyzz<F> acc = Xyzz<F>::inf();
+        for (int w = 0; w < 32; ++w) {
+            const uint32_t b = (uint32_t)(k[w / 8] >> (8 * (w % 8))) & 0xffu;
+            if (b) acc = acc.add(t[w * 256 + b]);
+        }
+        return acc;
+    }
+};
+
 typedef Xyzz<Fq> G1x;
 typedef Xyzz<Fq2> G2x;
 
