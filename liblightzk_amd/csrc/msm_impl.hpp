@@ -2,7 +2,6 @@
 // Fq2); included by msm_g1.hip and msm_g2.hip, one instantiation per
 // translation unit. Stages 1-4 and the batch layout are described in msm.hip.
 #pragma once
-#include <omp.h>
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -11,6 +10,7 @@
 #include "curve.hpp"
 #include "field29.hpp"
 #include "host_ec.hpp"
+#include "host_pool.hpp"
 #include "msm.hpp"
 
 namespace hzk {
@@ -1055,8 +1055,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
     std::vector<uint64_t> tbeg(count + 1, 0);
     for (size_t i = 0; i < count; ++i) tbeg[i + 1] = tbeg[i] + L.inst[i].W;
     std::vector<HostPt> T(tbeg[count]);
-#pragma omp parallel for schedule(dynamic, 1) if (tbeg[count] > 1)
-    for (long q = 0; q < (long)tbeg[count]; ++q) {
+    parallel_for((long)tbeg[count], [&](long q) {
         const size_t i = std::upper_bound(tbeg.begin(), tbeg.end(), (uint64_t)q) - tbeg.begin() - 1;
         const MsmInst& d = L.inst[i];
         const size_t j = (size_t)q - tbeg[i];
@@ -1074,11 +1073,10 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
             acc = acc.add(y);
         }
         T[q] = acc;
-    }
+    });
     // Horner over windows, instances in parallel
     std::vector<char> out(count * 3 * sizeof(F));
-#pragma omp parallel for schedule(dynamic, 1) if (count > 1)
-    for (long i = 0; i < (long)count; ++i) {
+    parallel_for((long)count, [&](long i) {
         const MsmInst& d = L.inst[i];
         HostPt acc = HostPt::inf();
         for (int j = (int)d.W - 1; j >= 0; --j) {
@@ -1088,7 +1086,7 @@ int msm_batch_run(hzk_ctx* ctx, const MsmJob* jobs, size_t count, int dests_on_h
             acc = acc.add(T[tbeg[i] + j]);
         }
         acc.store_jacobian(&out[i * 3 * sizeof(F)]);
-    }
+    });
     for (size_t i = 0; i < count; ++i) {
         if (dests_on_host) std::memcpy(jobs[i].dest, &out[i * 3 * sizeof(F)], 3 * sizeof(F));
         else HZK_HIP(ctx, hipMemcpyAsync(jobs[i].dest, &out[i * 3 * sizeof(F)], 3 * sizeof(F), hipMemcpyHostToDevice, st));

@@ -37,6 +37,7 @@
 #include <thread>
 #include <vector>
 #include "g16.hpp"
+#include "host_pool.hpp"
 
 using namespace hzk;
 
@@ -256,15 +257,11 @@ int stage_msm(hzk_server* s, hzk_ctx* ctx, std::vector<Request*>& rs) {
     for (Request* r : rs) reqs.push_back({s->circuits[r->circuit], r->dz, r->da, r->db, r->dc, r->rs, {}});
     // the key-and-(r, s) terms of every proof on the host while the device runs the MSMs
     std::vector<G16Fixed> fx(rs.size());
-    std::thread fixed([&] {
-#pragma omp parallel for schedule(dynamic, 1)
-        for (long i = 0; i < (long)rs.size(); ++i) fx[i] = g16_fixed_terms(reqs[i].g, rs[i]->rs);
-    });
+    std::thread fixed([&] { parallel_for((long)rs.size(), [&](long i) { fx[i] = g16_fixed_terms(reqs[i].g, rs[i]->rs); }); });
     int rc = g16_msm_batch(ctx, reqs.data(), reqs.size(), 0);
     fixed.join();
     if (rc) return rc;
-#pragma omp parallel for schedule(dynamic, 1)
-    for (long i = 0; i < (long)rs.size(); ++i) g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, fx[i], rs[i]->proof);
+    parallel_for((long)rs.size(), [&](long i) { g16_assemble(reqs[i].g, reqs[i].res, rs[i]->rs, fx[i], rs[i]->proof); });
     return 0;
 }
 
