@@ -3,17 +3,22 @@
 // host_ec.hpp), its seeded input streams (synth.cpp), the key-file header,
 // R1CS and trailer checks (keyfile_format.hpp) on a valid file and on every
 // corruption the loader must reject, and the oracle's CPU restatement
-// (oracle/*.cpp: field, curve, MSM, NTT, pairing, Groth16). Values are
-// compared with the oracle where both compute the same thing.
+// (oracle/*.cpp: field, curve, MSM, NTT, pairing, Groth16), the host
+// fixed-base tables (host_ec.hpp FixedBase) and the host worker pool
+// (host_pool.cpp) under concurrent and nested callers. Values are compared
+// with the oracle where both compute the same thing.
 //
 //   host_sanitize <scratch dir>     prints "host_sanitize ok" and exits 0
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 #include "../../liblightzk_amd/csrc/host_ec.hpp"
+#include "../../liblightzk_amd/csrc/host_pool.hpp"
 #include "../../liblightzk_amd/csrc/keyfile_format.hpp"
 
 extern "C" {
@@ -106,6 +111,50 @@ static void check_points_and_synth() {
     }
     CHECK(got1 == aff1, "host G1 k*G vs oracle");
     CHECK(got2 == aff2, "host G2 k*G vs oracle");
+
+    // byte-window tables of G: k*G by 32 table additions, k = 0 and k = 1 included
+    host::FixedBase<host::Fq> fb1;
+    host::FixedBase<host::Fq2> fb2;
+    fb1.build(host::g1_generator());
+    fb2.build(host::g2_generator());
+    k[0] = k[1] = k[2] = k[3] = 0;
+    k[4] = host::FR.one[0]; k[5] = host::FR.one[1]; k[6] = host::FR.one[2]; k[7] = host::FR.one[3];
+    orc_g1_mul_gen(k.data(), aff1.data(), n);
+    orc_g2_mul_gen(k.data(), aff2.data(), n);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t c[4];
+        host::Fr::load(&k[4 * i]).canonical(c);
+        fb1.mul(c).store_affine(&got1[8 * i]);
+        fb2.mul(c).store_affine(&got2[16 * i]);
+    }
+    CHECK(got1 == aff1, "host G1 fixed-base table vs oracle");
+    CHECK(got2 == aff2, "host G2 fixed-base table vs oracle");
+}
+
+// Four threads call parallel_for at once (as a server's pipelines do), with
+// ragged lengths and a nested call inside some indices; every index runs once.
+static void check_host_pool() {
+    std::atomic<long> bad{0};
+    std::vector<std::thread> callers;
+    for (int t = 0; t < 4; ++t)
+        callers.emplace_back([&bad, t] {
+            for (int rep = 0; rep < 40; ++rep) {
+                const long n = (rep * 37 + t * 11) % 97;   // 0 and 1 included
+                std::vector<int> hit(n, 0);
+                hzk::parallel_for(n, [&](long i) {
+                    ++hit[i];
+                    if (i % 16 == 0) {
+                        std::atomic<int> inner{0};
+                        hzk::parallel_for(5, [&](long) { ++inner; });
+                        if (inner != 5) ++bad;
+                    }
+                });
+                for (long i = 0; i < n; ++i)
+                    if (hit[i] != 1) ++bad;
+            }
+        });
+    for (auto& c : callers) c.join();
+    CHECK(bad == 0, "host pool runs every index exactly once");
 }
 
 static void check_oracle_msm_ntt_pairing() {
@@ -305,6 +354,7 @@ int main(int argc, char** argv) {
     }
     check_fields();
     check_points_and_synth();
+    check_host_pool();
     check_oracle_msm_ntt_pairing();
     check_groth16(6);
     check_keyfiles(argv[1]);
