@@ -589,7 +589,7 @@ def main():
     # the same ~255 dependent doublings would cost on one device lane
     d1, d2 = ctx.roofline_probe("dbl_chain_g1"), ctx.roofline_probe("dbl_chain_g2")
     out["msm_window_combine"] = {
-        "where": "host, after one D2H copy of the window sums (OpenMP over instances and windows)",
+        "where": "host, after one D2H copy of the window sums (the host worker pool over instances and windows)",
         "device_chain_doubling_us": {"g1": 1e6 / d1, "g2": 1e6 / d2},
         "device_horner_255_doublings_ms": {"g1": 255e3 / d1, "g2": 255e3 / d2},
         "source": "hzk_roofline_probe kinds 4/5: dependent doublings of the 29-bit accumulator, one wave",
