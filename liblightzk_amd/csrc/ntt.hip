@@ -41,13 +41,15 @@
 
 namespace hzk {
 
-// Tiles of 2^TB elements, TB = 9, 10 or 11, 2^(TB-3) threads each: a launch
-// takes the largest tile that still gives every CU two workgroups (a lone
-// 2^18 transform: 512 tiles of 2^9; batches and 2^20 and up: 2^11).
+// Tiles of 2^TB elements, TB = 9, 10 or 11, 2^(TB-LE) threads each holding
+// 2^LE elements: LE = 3 (a radix-8 group) where that gives every SIMD two
+// waves, else LE = 1 (launches too small for that, e.g. a lone 2^18
+// transform: two-element threads, 2048 waves). An LE = 3 launch takes the
+// largest tile that still gives every CU two workgroups (batches and 2^20 and
+// up: 2^11); an LE = 1 launch the largest that gives every CU one (2^18: 2^10).
 static constexpr int NTT_TILE_BITS = 11;   // largest tile
-static constexpr int NTT_EPT = 8;          // elements per thread
-template <int TB>
-constexpr int ntt_threads() { return (1 << TB) / NTT_EPT; }
+template <int TB, int LE = 3>
+constexpr int ntt_threads() { return (1 << TB) >> LE; }
 static constexpr int NTT_MAX_LOG = 26;
 
 // one pass of one transform over its tiles [tile_begin, next tile_begin)
@@ -106,124 +108,360 @@ struct NttTile {
     }
 };
 
-// the four butterflies of one stage over the thread's 8 elements v[i]
+// the 2^(LE-1) butterflies of one stage over the thread's 2^LE elements v[i]
 // (element i at tile index eb | (i << L)); the stage pairs bit IB of i
-template <int IB, bool DIT>
-HZK_DEV void ntt_stage8(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st) {
-    constexpr int P[4] = {0, 1, 2, 3};
-    int lo[4], hi[4];
+template <int IB, bool DIT, int LE>
+HZK_DEV void ntt_stage(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st) {
+    constexpr int H = 1 << (LE - 1);   // butterflies
+    int lo[H], hi[H];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int below = P[j] & ((1 << IB) - 1), above = (P[j] >> IB) << (IB + 1);
+    for (int j = 0; j < H; ++j) {
+        const int below = j & ((1 << IB) - 1), above = (j >> IB) << (IB + 1);
         lo[j] = above | below;
         hi[j] = lo[j] | (1 << IB);
     }
-    fr_t t[4];
+    fr_t t[H];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) t[j] = tw[T.texp(eb | ((uint32_t)lo[j] << L), st)];
+    for (int j = 0; j < H; ++j) t[j] = tw[T.texp(eb | ((uint32_t)lo[j] << L), st)];
     if (!DIT) {
-        fr_t d[4];
+        fr_t d[H];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < H; ++j) {
             d[j] = v[lo[j]] - v[hi[j]];
             v[lo[j]] = v[lo[j]] + v[hi[j]];
         }
-        fr_t::mul2(v[hi[0]], d[0], t[0], v[hi[1]], d[1], t[1]);
-        fr_t::mul2(v[hi[2]], d[2], t[2], v[hi[3]], d[3], t[3]);
-    } else {
-        fr_t p[4];
-        fr_t::mul2(p[0], v[hi[0]], t[0], p[1], v[hi[1]], t[1]);
-        fr_t::mul2(p[2], v[hi[2]], t[2], p[3], v[hi[3]], t[3]);
+        if constexpr (H == 1) v[hi[0]] = d[0] * t[0];
+        else
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < H; j += 2) fr_t::mul2(v[hi[j]], d[j], t[j], v[hi[j + 1]], d[j + 1], t[j + 1]);
+    } else {
+        fr_t p[H];
+        if constexpr (H == 1) p[0] = v[hi[0]] * t[0];
+        else
+#pragma unroll
+            for (int j = 0; j < H; j += 2) fr_t::mul2(p[j], v[hi[j]], t[j], p[j + 1], v[hi[j + 1]], t[j + 1]);
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
             v[hi[j]] = v[lo[j]] - p[j];
             v[lo[j]] = v[lo[j]] + p[j];
         }
     }
 }
 
-template <bool DIT>
-HZK_DEV void ntt_stage8_rt(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st, int ib) {
-    if (ib == 0) ntt_stage8<0, DIT>(v, T, tw, eb, L, st);
-    else if (ib == 1) ntt_stage8<1, DIT>(v, T, tw, eb, L, st);
-    else ntt_stage8<2, DIT>(v, T, tw, eb, L, st);
+template <bool DIT, int LE>
+HZK_DEV void ntt_stage_rt(fr_t* v, const NttTile& T, const fr_t* __restrict__ tw, uint32_t eb, int L, int st, int ib) {
+    if constexpr (LE == 1) ntt_stage<0, DIT, 1>(v, T, tw, eb, L, st);
+    else if constexpr (LE == 2) {
+        if (ib == 0) ntt_stage<0, DIT, 2>(v, T, tw, eb, L, st);
+        else ntt_stage<1, DIT, 2>(v, T, tw, eb, L, st);
+    } else {
+        if (ib == 0) ntt_stage<0, DIT, 3>(v, T, tw, eb, L, st);
+        else if (ib == 1) ntt_stage<1, DIT, 3>(v, T, tw, eb, L, st);
+        else ntt_stage<2, DIT, 3>(v, T, tw, eb, L, st);
+    }
 }
 
-// x[i] *= tab[i] for the thread's 8 elements, interleaved in pairs
-HZK_DEV void mul_table8(fr_t* v, const NttTile& T, const fr_t* __restrict__ tab, uint32_t eb, int L) {
+// x[i] *= tab[i] for the thread's 2^LE elements, interleaved in pairs
+template <int LE>
+HZK_DEV void mul_table(fr_t* v, const NttTile& T, const fr_t* __restrict__ tab, uint32_t eb, int L) {
 #pragma unroll
-    for (int i = 0; i < 8; i += 2) {
+    for (int i = 0; i < (1 << LE); i += 2) {
         const fr_t a = tab[T.gidx(eb | ((uint32_t)i << L))], b = tab[T.gidx(eb | ((uint32_t)(i + 1) << L))];
         fr_t::mul2(v[i], v[i], a, v[i + 1], v[i + 1], b);
     }
 }
 
-// One tile of a pass: stages in chunks of up to three. DIF runs local stages
+// One tile of a pass: stages in chunks of up to LE. DIF runs local stages
 // 0..R-1 from the first chunk (tile bits TB-1..) down; DIT runs them in
-// reverse. A thread's 8 elements for a chunk span the three tile bits
-// [L, L + 3) that contain the chunk's stage bits; its other tile bits come
+// reverse. A thread's 2^LE elements for a chunk span the LE tile bits
+// [L, L + LE) that contain the chunk's stage bits; its other tile bits come
 // from the thread index (low bits first, so a wave's lanes read consecutive
 // elements wherever L > 0).
-template <bool DIT, int TB>
+template <bool DIT, int TB, int LE = 3>
 HZK_DEV void ntt_tile(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const fr_t* __restrict__ pre,
                       const fr_t* __restrict__ post, int k, int s0, int R, int gbits, uint32_t b, uint4* sh) {
+    constexpr int E = 1 << LE;
     const NttTile T(k, s0, R, gbits, b);
     const uint32_t t = threadIdx.x;
-    const bool active = t < (1u << (T.TB - 3));
-    fr_t v[8];
-    const int nch = (R + 2) / 3;
+    const bool active = t < (1u << (T.TB - LE));
+    fr_t v[E];
+    const int nch = (R + LE - 1) / LE;
     uint32_t eb_prev = 0;
     int L_prev = 0;
     for (int c = 0; c < nch; ++c) {
         int stA, len;
         if (!DIT) {
-            stA = 3 * c;
-            len = min(3, R - stA);
+            stA = LE * c;
+            len = min(LE, R - stA);
         } else {
-            stA = max(0, R - 3 * (c + 1));
-            len = R - 3 * c - stA;
+            stA = max(0, R - LE * (c + 1));
+            len = R - LE * c - stA;
         }
         const int btop = gbits + R - 1 - stA, blo = btop - len + 1;
-        const int L = min(blo, T.TB - 3);
-        const uint32_t eb = (t & ((1u << L) - 1)) | ((t >> L) << (L + 3));
+        const int L = min(blo, T.TB - LE);
+        const uint32_t eb = (t & ((1u << L) - 1)) | ((t >> L) << (L + LE));
         if (c == 0) {
             if (active) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = x[T.gidx(eb | ((uint32_t)i << L))];
-                if (pre) mul_table8(v, T, pre, eb, L);
+                for (int i = 0; i < E; ++i) v[i] = x[T.gidx(eb | ((uint32_t)i << L))];
+                if (pre) mul_table<LE>(v, T, pre, eb, L);
             }
         } else {
             if (c > 1) __syncthreads();   // every thread has read the previous exchange
             if (active)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) lds_put<TB>(sh, eb_prev | ((uint32_t)i << L_prev), v[i]);
+                for (int i = 0; i < E; ++i) lds_put<TB>(sh, eb_prev | ((uint32_t)i << L_prev), v[i]);
             __syncthreads();
             if (active)
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = lds_get<TB>(sh, eb | ((uint32_t)i << L));
+                for (int i = 0; i < E; ++i) v[i] = lds_get<TB>(sh, eb | ((uint32_t)i << L));
         }
         if (active) {
             if (!DIT)
-                for (int st = stA; st < stA + len; ++st) ntt_stage8_rt<false>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
+                for (int st = stA; st < stA + len; ++st) ntt_stage_rt<false, LE>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
             else
-                for (int st = stA + len - 1; st >= stA; --st) ntt_stage8_rt<true>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
+                for (int st = stA + len - 1; st >= stA; --st) ntt_stage_rt<true, LE>(v, T, tw, eb, L, st, gbits + R - 1 - st - L);
         }
         eb_prev = eb;
         L_prev = L;
     }
     if (active) {
-        if (post) mul_table8(v, T, post, eb_prev, L_prev);
+        if (post) mul_table<LE>(v, T, post, eb_prev, L_prev);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[T.gidx(eb_prev | ((uint32_t)i << L_prev))] = v[i];
+        for (int i = 0; i < E; ++i) x[T.gidx(eb_prev | ((uint32_t)i << L_prev))] = v[i];
     }
 }
 
+// Fr in nine 29-bit limbs for the latency-bound two-element passes. The
+// boundary form's Montgomery product (field.hpp: eight 32-bit limbs, one
+// chain of 128 multiply-adds with carry folds) is a long dependency chain;
+// here the 17 product columns are independent sums (a 64-bit column cannot
+// overflow), and only the nine reduction digits form a chain of a few
+// instructions each, so one product finishes in a fraction of the time.
+// Montgomery radix 2^261: a product of values in the library's 2^256 form
+// stays in that form when one factor (a twiddle or table entry, canonical)
+// is first multiplied by 2^5. Values stay below 2r between operations.
+struct fr29 {
+    static constexpr uint32_t MASK = (1u << 29) - 1;
+    static constexpr uint32_t P[9] = {0x10000001, 0x1f0fac9f, 0x0e5c2450, 0x07d090f3, 0x1585d283,
+                                      0x02db40c0, 0x00a6e141, 0x0e5c2634, 0x0030644e};   // r
+    static constexpr uint32_t P2[9] = {0x00000002, 0x1e1f593f, 0x1cb848a1, 0x0fa121e6, 0x0b0ba506,
+                                       0x05b68181, 0x014dc282, 0x1cb84c68, 0x0060c89c};  // 2r
+    static constexpr uint32_t M0 = 0x0fffffff;   // -r^-1 mod 2^29
+    uint32_t l[9];
+
+    // 8 x 32-bit limbs (a value < 2^256) -> 9 x 29-bit limbs
+    static HZK_DEV fr29 unpack(const fr_t& x) {
+        const uint32_t* w = x.l;
+        fr29 r;
+        r.l[0] = w[0] & MASK;
+        r.l[1] = ((w[0] >> 29) | (w[1] << 3)) & MASK;
+        r.l[2] = ((w[1] >> 26) | (w[2] << 6)) & MASK;
+        r.l[3] = ((w[2] >> 23) | (w[3] << 9)) & MASK;
+        r.l[4] = ((w[3] >> 20) | (w[4] << 12)) & MASK;
+        r.l[5] = ((w[4] >> 17) | (w[5] << 15)) & MASK;
+        r.l[6] = ((w[5] >> 14) | (w[6] << 18)) & MASK;
+        r.l[7] = ((w[6] >> 11) | (w[7] << 21)) & MASK;
+        r.l[8] = w[7] >> 8;
+        return r;
+    }
+    // normalized limbs of a value < 2^256 -> 8 x 32-bit limbs
+    HZK_DEV fr_t pack() const {
+        fr_t x;
+        x.l[0] = l[0] | (l[1] << 29);
+        x.l[1] = (l[1] >> 3) | (l[2] << 26);
+        x.l[2] = (l[2] >> 6) | (l[3] << 23);
+        x.l[3] = (l[3] >> 9) | (l[4] << 20);
+        x.l[4] = (l[4] >> 12) | (l[5] << 17);
+        x.l[5] = (l[5] >> 15) | (l[6] << 14);
+        x.l[6] = (l[6] >> 18) | (l[7] << 11);
+        x.l[7] = (l[7] >> 21) | (l[8] << 8);
+        return x;
+    }
+    // 32 x (a value < 2^256): a 2^256-form factor made a 2^261-form one
+    HZK_DEV fr29 times32() const {
+        fr29 r;
+        r.l[0] = (l[0] << 5) & MASK;
+#pragma unroll
+        for (int i = 1; i < 9; ++i) r.l[i] = ((l[i] << 5) | (l[i - 1] >> 24)) & MASK;
+        return r;
+    }
+    // a + b, normalized (a + b < 2^256)
+    static HZK_DEV fr29 add(const fr29& a, const fr29& b) {
+        fr29 r;
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const uint32_t v = a.l[i] + b.l[i] + c;
+            r.l[i] = v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+    // a - b + 2r for b <= 2r, normalized
+    static HZK_DEV fr29 sub2(const fr29& a, const fr29& b) {
+        fr29 r;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int32_t v = (int32_t)(a.l[i] + P2[i] - b.l[i]) + c;
+            r.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+    // a normalized value below 2^261 brought below 2r: subtract q r with
+    // q = floor(l8 / (r8 + 1)) <= floor(value / r); what remains is below
+    // (r8 + 1 + q) 2^232 < 2r
+    HZK_DEV fr29 reduce() const {
+        const uint32_t q = l[8] / (P[8] + 1);
+        fr29 r;
+        int64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int64_t v = (int64_t)l[i] - (int64_t)q * P[i] + c;
+            r.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return r;
+    }
+    // a value below 2r -> below r
+    HZK_DEV fr29 canonical() const {
+        fr29 d;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const int32_t v = (int32_t)l[i] - (int32_t)P[i] + c;
+            d.l[i] = (uint32_t)v & MASK;
+            c = v >> 29;
+        }
+        return c < 0 ? *this : d;
+    }
+    // a b 2^-261 mod r, below 2r when the bounds of a and b (in units of r)
+    // multiply to less than 2^261 / r > 169; normalized inputs. A column holds
+    // at most 9 limb products and 9 reduction products (< 2^58 each) and a
+    // carry: < 2^62.2.
+    static HZK_DEV fr29 mul(const fr29& a, const fr29& b) {
+        uint64_t c[18];
+#pragma unroll
+        for (int k = 0; k < 17; ++k) {
+            uint64_t s = 0;
+#pragma unroll
+            for (int j = (k > 8 ? k - 8 : 0); j <= (k < 8 ? k : 8); ++j) s += (uint64_t)a.l[j] * b.l[k - j];
+            c[k] = s;
+        }
+        c[17] = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const uint32_t m = ((uint32_t)c[k] * M0) & MASK;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) c[k + i] += (uint64_t)m * P[i];
+            c[k + 1] += c[k] >> 29;
+        }
+        fr29 r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            r.l[i] = (uint32_t)c[9 + i] & MASK;
+            c[10 + i] += c[9 + i] >> 29;
+        }
+        r.l[8] = (uint32_t)c[17];
+        return r;
+    }
+};
+
+// The same pass with two elements per thread (one butterfly per stage, an
+// LDS exchange between stages) in fr29 arithmetic, for lone transforms that
+// are latency-bound: each stage's twiddle is loaded one stage ahead, so no
+// butterfly waits for a global load.
 template <bool DIT, int TB>
-__global__ void __launch_bounds__(ntt_threads<TB>()) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw,
-                                                                const fr_t* __restrict__ pre, const fr_t* __restrict__ post,
-                                                                int k, int s0, int R, int gbits) {
+HZK_DEV void ntt_tile_r2(fr_t* __restrict__ x, const fr_t* __restrict__ tw, const fr_t* __restrict__ pre,
+                         const fr_t* __restrict__ post, int k, int s0, int R, int gbits, uint32_t b, uint4* sh) {
+    const NttTile T(k, s0, R, gbits, b);
+    const uint32_t t = threadIdx.x;
+    const bool active = t < (1u << (T.TB - 1));
+    // executed step i runs local stage st(i) on tile bit L(i) = gbits + R - 1 - st(i);
+    // the thread's pair is (eb(i), eb(i) | 2^L(i))
+    auto stage_of = [&](int i) { return DIT ? R - 1 - i : i; };
+    auto bit_of = [&](int i) { return gbits + R - 1 - stage_of(i); };
+    auto pair_base = [&](int L) { return (t & ((1u << L) - 1)) | ((t >> L) << (L + 1)); };
+    fr29 v0, v1;
+    fr_t w, wn;
+    uint32_t eb = 0;
+    int L = 0;
+    bool lds_used = false;
+    if (active) w = tw[T.texp(pair_base(bit_of(0)), stage_of(0))];
+    for (int i = 0; i < R; ++i) {
+        const int Li = bit_of(i);
+        const uint32_t ebi = pair_base(Li);
+        if (active && i + 1 < R) wn = tw[T.texp(pair_base(bit_of(i + 1)), stage_of(i + 1))];
+        if (i == 0) {
+            if (active) {
+                v0 = fr29::unpack(x[T.gidx(ebi)]).reduce();
+                v1 = fr29::unpack(x[T.gidx(ebi | (1u << Li))]).reduce();
+                if (pre) {
+                    v0 = fr29::mul(v0, fr29::unpack(pre[T.gidx(ebi)]).times32());
+                    v1 = fr29::mul(v1, fr29::unpack(pre[T.gidx(ebi | (1u << Li))]).times32());
+                }
+            }
+        } else if (min(L, Li) < 6) {
+            // the pair bits of this stage and the last differ by one; the
+            // threads that trade elements differ in thread bit pb = min(L, Li)
+            // and share a wave: swap one element across lanes, no barrier.
+            // Thread bit pb = b keeps v_b and receives the partner's element
+            // into v_(1-b) (both for DIF, pb = Li, and DIT, pb = L).
+            const int pb = min(L, Li);
+            const bool hi = (t >> pb) & 1;
+            fr29 send = hi ? v0 : v1, got;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) got.l[q] = (uint32_t)__shfl_xor((int)send.l[q], 1 << pb, 64);
+            if (hi) v0 = got;
+            else v1 = got;
+        } else {
+            if (lds_used) __syncthreads();   // every thread has read the previous LDS exchange
+            if (active) {
+                lds_put<TB>(sh, eb, v0.pack());
+                lds_put<TB>(sh, eb | (1u << L), v1.pack());
+            }
+            __syncthreads();
+            if (active) {
+                v0 = fr29::unpack(lds_get<TB>(sh, ebi));
+                v1 = fr29::unpack(lds_get<TB>(sh, ebi | (1u << Li)));
+            }
+            lds_used = true;
+        }
+        if (active) {
+            const fr29 w32 = fr29::unpack(w).times32();   // canonical twiddle: < 32r
+            if (!DIT) {
+                const fr29 d = fr29::sub2(v0, v1);         // < 4r
+                v0 = fr29::add(v0, v1).reduce();
+                v1 = fr29::mul(d, w32);                    // (4)(32) < 169: < 2r
+            } else {
+                const fr29 p = fr29::mul(v1, w32);         // (2)(32)
+                v1 = fr29::sub2(v0, p).reduce();
+                v0 = fr29::add(v0, p).reduce();
+            }
+        }
+        w = wn;
+        eb = ebi;
+        L = Li;
+    }
+    if (active) {
+        if (post) {
+            v0 = fr29::mul(v0, fr29::unpack(post[T.gidx(eb)]).times32());
+            v1 = fr29::mul(v1, fr29::unpack(post[T.gidx(eb | (1u << L))]).times32());
+        }
+        x[T.gidx(eb)] = v0.canonical().pack();
+        x[T.gidx(eb | (1u << L))] = v1.canonical().pack();
+    }
+}
+
+template <bool DIT, int TB, int LE>
+__global__ void __launch_bounds__((ntt_threads<TB, LE>())) k_ntt_pass(fr_t* __restrict__ x, const fr_t* __restrict__ tw,
+                                                                    const fr_t* __restrict__ pre, const fr_t* __restrict__ post,
+                                                                    int k, int s0, int R, int gbits) {
     __shared__ uint4 sh[2 << TB];
-    ntt_tile<DIT, TB>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
+    if constexpr (LE == 1) ntt_tile_r2<DIT, TB>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
+    else ntt_tile<DIT, TB, LE>(x, tw, pre, post, k, s0, R, gbits, blockIdx.x, sh);
 }
 
 // transforms of 2 and 4 points (too small for an 8-element group): one thread each
@@ -269,14 +507,19 @@ HZK_DEV uint32_t find_elem(const NttElemDesc* d, uint32_t nd, uint64_t gid) {
     return lo;
 }
 
-template <int TB>
-__global__ void __launch_bounds__(ntt_threads<TB>()) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
+template <int TB, int LE>
+__global__ void __launch_bounds__((ntt_threads<TB, LE>())) k_ntt_pass_batch(const NttPassDesc* __restrict__ pd, uint32_t npd) {
     __shared__ uint4 sh[2 << TB];
     const uint32_t i = find_desc(pd, npd, blockIdx.x);
     const NttPassDesc& d = pd[i];
     const uint32_t b = (uint32_t)(blockIdx.x - d.tile_begin);
-    if (d.dit) ntt_tile<true, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
-    else ntt_tile<false, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    if constexpr (LE == 1) {
+        if (d.dit) ntt_tile_r2<true, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+        else ntt_tile_r2<false, TB>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    } else {
+        if (d.dit) ntt_tile<true, TB, LE>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+        else ntt_tile<false, TB, LE>(d.x, d.tw, d.pre, d.post, d.k, d.s0, d.R, d.gbits, b, sh);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_bitrev_batch(const NttElemDesc* __restrict__ ed, uint32_t ne, uint64_t total) {
@@ -482,13 +725,39 @@ static int tile_bits_for(hzk_ctx* ctx, uint64_t elems) {
     return TB;
 }
 
-template <bool DIT, int TB>
+// elements per thread (2^LE) of a launch over `elems` elements: 8 while
+// that still gives every SIMD two waves, else 2 (more threads and more
+// exchanges: such transforms are latency-bound, not issue-bound; 4 measured
+// slower than 2)
+static int elem_bits_for(hzk_ctx* ctx, uint64_t elems) {
+    const uint64_t want = 2ull * 4 * 64 * (uint64_t)ctx->num_cus;   // threads for two waves per SIMD
+    return (elems >> 3) >= want ? 3 : 1;
+}
+// tile bits of a launch with 2^LE elements per thread: the largest tile that
+// gives every CU two workgroups (LE = 3) or one workgroup of 2^(TB-1)
+// threads (LE = 1: already two waves per SIMD, in fewer passes)
+static int tile_bits_le(hzk_ctx* ctx, uint64_t elems, int le) {
+    int TB = tile_bits_for(ctx, elems);
+    if (le == 1)
+        while (TB < NTT_TILE_BITS && (elems >> (TB + 1)) >= (uint64_t)ctx->num_cus) ++TB;
+    return TB;
+}
+
+template <bool DIT, int TB, int LE>
 static void launch_pass(hipStream_t st, unsigned tiles, fr_t* x, const fr_t* tw, const fr_t* pp, const fr_t* qq, int k, int3 pl) {
-    hipLaunchKernelGGL((k_ntt_pass<DIT, TB>), dim3(tiles), dim3(ntt_threads<TB>()), 0, st, x, tw, pp, qq, k, pl.x, pl.y, pl.z);
+    hipLaunchKernelGGL((k_ntt_pass<DIT, TB, LE>), dim3(tiles), dim3(ntt_threads<TB, LE>()), 0, st, x, tw, pp, qq, k, pl.x, pl.y,
+                       pl.z);
+}
+template <bool DIT, int TB>
+static void launch_pass_le(int le, hipStream_t st, unsigned tiles, fr_t* x, const fr_t* tw, const fr_t* pp, const fr_t* qq, int k,
+                           int3 pl) {
+    if (le == 1) launch_pass<DIT, TB, 1>(st, tiles, x, tw, pp, qq, k, pl);
+    else launch_pass<DIT, TB, 3>(st, tiles, x, tw, pp, qq, k, pl);
 }
 
 static void ntt_passes(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, int dit, const fr_t* pre, const fr_t* post) {
-    const int TB = std::min(k, tile_bits_for(ctx, (uint64_t)1 << k));
+    const int le = elem_bits_for(ctx, (uint64_t)1 << k);
+    const int TB = std::min(k, tile_bits_le(ctx, (uint64_t)1 << k, le));
     std::vector<int3> plan;
     plan_passes(k, std::max(TB, 3), plan);
     if (dit) std::reverse(plan.begin(), plan.end());
@@ -500,14 +769,14 @@ static void ntt_passes(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, int dit, co
         // the kernel's tile holds 2^(R + gbits) <= 2^TB elements
         const int tb = pl.y + pl.z;
         if (tb <= 9) {
-            if (dit) launch_pass<true, 9>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
-            else launch_pass<false, 9>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            if (dit) launch_pass_le<true, 9>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass_le<false, 9>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
         } else if (tb == 10) {
-            if (dit) launch_pass<true, 10>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
-            else launch_pass<false, 10>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            if (dit) launch_pass_le<true, 10>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass_le<false, 10>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
         } else {
-            if (dit) launch_pass<true, 11>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
-            else launch_pass<false, 11>(ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            if (dit) launch_pass_le<true, 11>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
+            else launch_pass_le<false, 11>(le, ctx->stream, tiles, x, tw, pp, qq, k, pl);
         }
     }
 }
@@ -597,7 +866,8 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
     size_t rounds = 0;
     uint64_t all = 0;
     for (size_t i = 0; i < count; ++i) all += jobs[i].deg;
-    const int TB = tile_bits_for(ctx, all);
+    const int LE = elem_bits_for(ctx, all);
+    const int TB = tile_bits_le(ctx, all, LE);
     for (size_t i = 0; i < count; ++i) {
         HZK_CHECK(ctx, jobs[i].x && jobs[i].kind >= 0 && jobs[i].kind <= 6, HZK_ERR_INVALID, "ntt batch: bad job %zu", i);
         int k = log2_exact(jobs[i].deg);
@@ -669,21 +939,23 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         }
         for (size_t r = 0; r < rounds; ++r) {
             uint32_t np = (uint32_t)(round_begin[r + 1] - round_begin[r]);
-            if (TB == 9)
-                hipLaunchKernelGGL(k_ntt_pass_batch<9>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<9>()), 0, st,
-                                   dpd + round_begin[r], np);
-            else if (TB == 10)
-                hipLaunchKernelGGL(k_ntt_pass_batch<10>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<10>()), 0, st,
-                                   dpd + round_begin[r], np);
-            else
-                hipLaunchKernelGGL(k_ntt_pass_batch<11>, dim3((unsigned)round_tiles[r]), dim3(ntt_threads<11>()), 0, st,
-                                   dpd + round_begin[r], np);
+            const dim3 grid((unsigned)round_tiles[r]);
+            const NttPassDesc* rp = dpd + round_begin[r];
+            if (LE == 1) {
+                if (TB == 9) hipLaunchKernelGGL((k_ntt_pass_batch<9, 1>), grid, dim3(ntt_threads<9, 1>()), 0, st, rp, np);
+                else if (TB == 10) hipLaunchKernelGGL((k_ntt_pass_batch<10, 1>), grid, dim3(ntt_threads<10, 1>()), 0, st, rp, np);
+                else hipLaunchKernelGGL((k_ntt_pass_batch<11, 1>), grid, dim3(ntt_threads<11, 1>()), 0, st, rp, np);
+            } else {
+                if (TB == 9) hipLaunchKernelGGL((k_ntt_pass_batch<9, 3>), grid, dim3(ntt_threads<9, 3>()), 0, st, rp, np);
+                else if (TB == 10) hipLaunchKernelGGL((k_ntt_pass_batch<10, 3>), grid, dim3(ntt_threads<10, 3>()), 0, st, rp, np);
+                else hipLaunchKernelGGL((k_ntt_pass_batch<11, 3>), grid, dim3(ntt_threads<11, 3>()), 0, st, rp, np);
+            }
         }
         if (elems)
             hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)eds.size(), elems);
     };
     if (capture) {
-        std::string key = "ntt:" + std::to_string(TB) + ":";
+        std::string key = "ntt:" + std::to_string(TB) + "/" + std::to_string(LE) + ":";
         for (size_t i = 0; i < count; ++i) {
             key += std::to_string(ks[i]) + "/" + std::to_string(jobs[i].kind) + ";";
             if (ks[i] > 0 && ks[i] < 3) key += std::to_string((uintptr_t)jobs[i].x) + ";";   // launched with direct arguments
