@@ -69,8 +69,8 @@ def test_ntt_delta_and_constant(ctx, orc):
     assert np.array_equal(d.download(np.uint64, 4 * n), want)
 
 
-# a lone transform's threads hold 2 elements up to 2^18, 4 at 2^19 and 8 from
-# 2^20 on; each form is compared with the oracle
+# a lone transform's threads hold 2 elements up to 2^19 and 8 from 2^20 on;
+# both forms are compared with the oracle
 @pytest.mark.parametrize("log_n", [18, 19, 20, 22])
 def test_ntt_large(ctx, orc, log_n):
     n = 1 << log_n
