@@ -544,6 +544,48 @@ __global__ void __launch_bounds__(256) k_bitrev_batch(const NttElemDesc* __restr
 }
 
 // bit-reverse permutation (k bits) fused with an optional scale factor
+// The same for k >= 8 through LDS in 16 x 16 blocks: index i = (a, m, b)
+// with a its top 4 bits, b its low 4 and m the k - 8 bits between, so
+// rev(i) = (rev b, rev m, rev a): block m (every a, b) and block rev(m) trade
+// places, transposed, and every global access is a run of 16 consecutive
+// elements. One workgroup per m; the one with m > rev(m) has nothing to do.
+// factor scales every element; post[p] the one landing at p.
+HZK_DEV void bitrev_tile(fr_t* __restrict__ x, int k, uint32_t m, const fr_t* __restrict__ factor,
+                         const fr_t* __restrict__ post, fr_t (*blk)[16 * 17]) {
+    const int mb = k - 8;
+    const uint32_t mr = mb ? __brev(m) >> (32 - mb) : 0;
+    if (m > mr) return;
+    const uint32_t t = threadIdx.x, a = t >> 4, b = t & 15;
+    const size_t stride_a = (size_t)1 << (k - 4);
+    auto at = [&](uint32_t aa, uint32_t mm, uint32_t bb) { return (size_t)aa * stride_a + ((size_t)mm << 4) + bb; };
+    fr_t v0 = x[at(a, m, b)], v1 = x[at(a, mr, b)];
+    if (factor) {
+        const fr_t f = *factor;
+        fr_t::mul2(v0, v0, f, v1, v1, f);
+    }
+    blk[0][a * 17 + b] = v0;   // rows padded to 17 elements
+    blk[1][a * 17 + b] = v1;
+    __syncthreads();
+    // destination (a, mr, b) takes source (rev b, m, rev a), and (a, m, b) takes (rev b, mr, rev a)
+    const uint32_t src = (__brev(b) >> 28) * 17 + (__brev(a) >> 28);
+    const size_t d0 = at(a, mr, b), d1 = at(a, m, b);
+    v0 = blk[0][src];
+    v1 = blk[1][src];
+    if (post) fr_t::mul2(v0, v0, post[d0], v1, v1, post[d1]);
+    x[d0] = v0;
+    if (m != mr) x[d1] = v1;
+}
+__global__ void __launch_bounds__(256) k_bitrev_tiles(fr_t* __restrict__ x, int k, const fr_t* __restrict__ factor) {
+    __shared__ fr_t blk[2][16 * 17];
+    bitrev_tile(x, k, blockIdx.x, factor, nullptr, blk);
+}
+// batched: transforms of 2^8 points and more, begin counting blocks (2^(k-8) each)
+__global__ void __launch_bounds__(256) k_bitrev_tiles_batch(const NttElemDesc* __restrict__ ed, uint32_t ne) {
+    __shared__ fr_t blk[2][16 * 17];
+    const NttElemDesc& d = ed[find_elem(ed, ne, blockIdx.x)];
+    bitrev_tile(d.x, d.k, (uint32_t)(blockIdx.x - d.begin), d.factor, d.post, blk);
+}
+
 __global__ void __launch_bounds__(256) k_bitrev_scale(fr_t* __restrict__ x, size_t n, int k, const fr_t* __restrict__ factor) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -798,7 +840,8 @@ int ntt_run(hzk_ctx* ctx, fr_t* x, int k, const fr_t* tw, const fr_t* factor) {
     }
     {
         TimedLaunch tl(ctx, "ntt_bitrev");
-        hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, k, factor);
+        if (k >= 8) hipLaunchKernelGGL(k_bitrev_tiles, dim3(1u << (k - 8)), dim3(256), 0, st, x, k, factor);
+        else hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, k, factor);
     }
     HZK_HIP(ctx, hipGetLastError());
     return 0;
@@ -828,8 +871,11 @@ int coset_ntt(hzk_ctx* ctx, fr_t* x, size_t deg, int inverse) {
                 ntt_passes(ctx, x, k, t->tw[0], 0, t->coset[0], nullptr);
             }
             TimedLaunch tl(ctx, "ntt_bitrev");
-            hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(deg, 256)), dim3(256), 0, ctx->stream, x, deg, k,
-                               (const fr_t*)nullptr);
+            if (k >= 8)
+                hipLaunchKernelGGL(k_bitrev_tiles, dim3(1u << (k - 8)), dim3(256), 0, ctx->stream, x, k, (const fr_t*)nullptr);
+            else
+                hipLaunchKernelGGL(k_bitrev_scale, dim3(blocks_for(deg, 256)), dim3(256), 0, ctx->stream, x, deg, k,
+                                   (const fr_t*)nullptr);
             HZK_HIP(ctx, hipGetLastError());
             return 0;
         }
@@ -906,27 +952,38 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         round_tiles[r] = tiles;
     }
     round_begin[rounds] = pds.size();
-    std::vector<NttElemDesc> eds;
-    uint64_t elems = 0;
+    // the permutation of kinds 0..3: element-wise below 2^8 points (begin in
+    // elements), else in 16 x 16 blocks (a second table, begin in blocks)
+    std::vector<NttElemDesc> eds, bds;
+    uint64_t elems = 0, blocks = 0;
     for (size_t i = 0; i < count; ++i) {
         const int kind = jobs[i].kind;
         if (kind > 3) continue;
-        eds.push_back({(fr_t*)jobs[i].x, (kind & 1) ? tabs[i]->ninv : nullptr, kind == 3 ? tabs[i]->coset[1] : nullptr, ks[i],
-                       elems});
-        elems += jobs[i].deg;
+        const NttElemDesc d{(fr_t*)jobs[i].x, (kind & 1) ? tabs[i]->ninv : nullptr, kind == 3 ? tabs[i]->coset[1] : nullptr,
+                            ks[i], ks[i] >= 8 ? blocks : elems};
+        if (ks[i] >= 8) {
+            bds.push_back(d);
+            blocks += (uint64_t)1 << (ks[i] - 8);
+        } else {
+            eds.push_back(d);
+            elems += jobs[i].deg;
+        }
     }
     // descriptors to the device
-    size_t pbytes = pds.size() * sizeof(NttPassDesc), ebytes = eds.size() * sizeof(NttElemDesc);
-    size_t eoff = (pbytes + 255) & ~size_t(255);
-    std::vector<char> host(eoff + ebytes);
+    size_t pbytes = pds.size() * sizeof(NttPassDesc), ebytes = eds.size() * sizeof(NttElemDesc),
+           bbytes = bds.size() * sizeof(NttElemDesc);
+    size_t eoff = (pbytes + 255) & ~size_t(255), boff = (eoff + ebytes + 255) & ~size_t(255);
+    std::vector<char> host(boff + bbytes);
     std::memcpy(host.data(), pds.data(), pbytes);
     std::memcpy(host.data() + eoff, eds.data(), ebytes);
+    std::memcpy(host.data() + boff, bds.data(), bbytes);
     const void* dev = nullptr;
     int rc = upload_desc(ctx, host.data(), host.size(), &dev);
     if (rc) return rc;
     hipStream_t st = ctx->stream;
     const NttPassDesc* dpd = (const NttPassDesc*)dev;
     const NttElemDesc* ded = (const NttElemDesc*)((const char*)dev + eoff);
+    const NttElemDesc* dbd = (const NttElemDesc*)((const char*)dev + boff);
 
     auto stages = [&]() {
         for (size_t i = 0; i < count; ++i) {   // 2- and 4-point transforms
@@ -953,6 +1010,8 @@ int ntt_batch(hzk_ctx* ctx, const NttJob* jobs, size_t count, int capture) {
         }
         if (elems)
             hipLaunchKernelGGL(k_bitrev_batch, dim3(blocks_for(elems, 256)), dim3(256), 0, st, ded, (uint32_t)eds.size(), elems);
+        if (blocks)
+            hipLaunchKernelGGL(k_bitrev_tiles_batch, dim3((unsigned)blocks), dim3(256), 0, st, dbd, (uint32_t)bds.size());
     };
     if (capture) {
         std::string key = "ntt:" + std::to_string(TB) + "/" + std::to_string(LE) + ":";
