@@ -370,7 +370,7 @@ def _pct(xs, q):
     return xs[min(len(xs) - 1, int(len(xs) * q))]
 
 
-def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=2, levels=1,
+def groth16_served(logs, weights, requests, rate_mult, max_batch, seed=5, pipelines=3, levels=1,
                    modes=("sequential", "continuous"), under_load_requests=150):
     """Mixed-size request stream served two ways by one hzk_server holding all
     circuits' keys: continuous batching (mode 0) and the plain sequential
@@ -492,7 +492,7 @@ def main():
                     help="length of the served request stream (a finite Poisson stream of this many requests)")
     ap.add_argument("--serve-rate-mult", type=float, default=4.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
-    ap.add_argument("--serve-pipelines", type=int, default=2)
+    ap.add_argument("--serve-pipelines", type=int, default=3)
     ap.add_argument("--serve-modes", default="sequential,continuous",
                     help="served-leg modes to run (both are needed for the speedup figure)")
     ap.add_argument("--no-batched", action="store_true")

@@ -262,7 +262,7 @@ int hzk_g16_load(hzk_ctx* ctx, const char* path, hzk_g16** out);
 typedef struct hzk_server hzk_server;
 int hzk_server_create(int device, size_t mem_budget, int max_batch, int mode, hzk_server** out);
 /* number of pipelines (worker thread + two streams + MSM scratch each) sharing
- * the admission queue, 1..8, before hzk_server_start; default 2. In mode 1
+ * the admission queue, 1..8, before hzk_server_start; default 3. In mode 1
  * only the first pipeline proves. */
 int hzk_server_set_pipelines(hzk_server* s, int n);
 void hzk_server_destroy(hzk_server* s);

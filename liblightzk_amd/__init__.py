@@ -362,7 +362,7 @@ class ProofServer:
     returns a request id; wait() returns (status, proof, latency_s) with status
     1 done, 0 still running, < 0 failed (an HZK_ERR_* code)."""
 
-    def __init__(self, device=0, mem_budget=0, max_batch=64, mode="continuous", pipelines=2):
+    def __init__(self, device=0, mem_budget=0, max_batch=64, mode="continuous", pipelines=3):
         h = ctypes.c_void_p()
         m = {"continuous": 0, "sequential": 1}[mode]
         rc = load().hzk_server_create(device, mem_budget, max_batch, m, ctypes.byref(h))
