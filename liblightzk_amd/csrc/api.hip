@@ -140,7 +140,6 @@ int ctx_aux(hzk_ctx* ctx, hzk_ctx** out) {
     if (!ctx->aux) {
         int rc = hzk_ctx_create(ctx->device, nullptr, &ctx->aux);
         if (rc) return set_error(ctx, rc, "auxiliary context: creation failed");
-        
     }
     std::memcpy(ctx->aux->tune, ctx->tune, sizeof ctx->tune);
     ctx->aux->timing = ctx->timing;
