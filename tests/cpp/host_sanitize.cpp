@@ -19,6 +19,15 @@
 #include <vector>
 #include "../../liblightzk_amd/csrc/host_ec.hpp"
 #include "../../liblightzk_amd/csrc/host_pool.hpp"
+// the NTT's 29-bit Fr arithmetic compiled for the host: it needs only the
+// eight-limb value type and a function qualifier
+namespace hzk {
+struct fr_t {
+    uint32_t l[8];
+};
+}  // namespace hzk
+#define HZK_DEV inline
+#include "../../liblightzk_amd/csrc/fr29.hpp"
 #include "../../liblightzk_amd/csrc/keyfile_format.hpp"
 
 extern "C" {
@@ -129,6 +138,69 @@ static void check_points_and_synth() {
     }
     CHECK(got1 == aff1, "host G1 fixed-base table vs oracle");
     CHECK(got2 == aff2, "host G2 fixed-base table vs oracle");
+}
+
+// fr29 (the NTT's two-element passes) against the oracle's Fr: the product
+// of a value below 2^256 (brought below 2r) with a canonical twiddle scaled
+// by 2^5 is the 2^256-form Montgomery product; sums and differences are
+// congruent and stay below 2r; canonical() gives the oracle's bytes.
+static void check_fr29() {
+    const size_t n = 512;
+    U a(4 * n), b(4 * n), prod(4 * n), sum(4 * n), dif(4 * n);
+    orc_gen_fr(91, 1, n, 0, a.data());
+    orc_gen_fr(91, 2, n, 0, b.data());
+    // edge values: 0, 1 (Montgomery), r - 1 and, for a, the largest 2^256 - 1
+    for (int q = 0; q < 4; ++q) {
+        a[q] = b[q] = 0;
+        a[4 + q] = b[4 + q] = host::FR.one[q];
+        a[8 + q] = host::FR.p[q] - (q == 0);
+        b[8 + q] = host::FR.p[q] - (q == 0);
+        a[12 + q] = ~0ull;
+    }
+    U bc(b), ac(a);
+    for (int q = 0; q < 4; ++q) ac[12 + q] = 0;   // the oracle's operands stay canonical
+    orc_fr_op(0, ac.data(), bc.data(), prod.data(), n);
+    orc_fr_op(1, ac.data(), bc.data(), sum.data(), n);
+    orc_fr_op(2, ac.data(), bc.data(), dif.data(), n);
+    for (size_t i = 0; i < n; ++i) {
+        if (i == 3) continue;   // 2^256 - 1: checked below against the reduced value
+        fr_t x, y;
+        for (int w = 0; w < 4; ++w) {
+            x.l[2 * w] = (uint32_t)a[4 * i + w], x.l[2 * w + 1] = (uint32_t)(a[4 * i + w] >> 32);
+            y.l[2 * w] = (uint32_t)b[4 * i + w], y.l[2 * w + 1] = (uint32_t)(b[4 * i + w] >> 32);
+        }
+        const fr29 xa = fr29::unpack(x).reduce(), yb = fr29::unpack(y).reduce();
+        const fr_t p = fr29::mul(xa, fr29::unpack(y).times32()).canonical().pack();
+        const fr_t s = fr29::add(xa, yb).reduce().canonical().pack();
+        const fr_t d = fr29::sub2(xa, yb).reduce().canonical().pack();
+        for (int w = 0; w < 4; ++w) {
+            CHECK((p.l[2 * w] | (uint64_t)p.l[2 * w + 1] << 32) == prod[4 * i + w], "fr29 product vs oracle");
+            CHECK((s.l[2 * w] | (uint64_t)s.l[2 * w + 1] << 32) == sum[4 * i + w], "fr29 sum vs oracle");
+            CHECK((d.l[2 * w] | (uint64_t)d.l[2 * w + 1] << 32) == dif[4 * i + w], "fr29 difference vs oracle");
+        }
+    }
+    // 2^256 - 1 (not canonical) times one: reduce() then the product gives (2^256 - 1) mod r
+    fr_t x, one;
+    for (int w = 0; w < 8; ++w) x.l[w] = ~0u;
+    for (int w = 0; w < 4; ++w) one.l[2 * w] = (uint32_t)host::FR.one[w], one.l[2 * w + 1] = (uint32_t)(host::FR.one[w] >> 32);
+    const fr_t got = fr29::mul(fr29::unpack(x).reduce(), fr29::unpack(one).times32()).canonical().pack();
+    // expected: (2^256 - 1) mod r, by repeated subtraction of r on 64-bit words
+    uint64_t e[4] = {~0ull, ~0ull, ~0ull, ~0ull};
+    for (;;) {
+        bool ge = true;
+        for (int w = 3; w >= 0; --w) {
+            if (e[w] != host::FR.p[w]) { ge = e[w] > host::FR.p[w]; break; }
+        }
+        if (!ge) break;
+        unsigned __int128 bw = 0;
+        for (int w = 0; w < 4; ++w) {
+            const unsigned __int128 v = (unsigned __int128)e[w] - host::FR.p[w] - (uint64_t)bw;
+            e[w] = (uint64_t)v;
+            bw = (v >> 64) ? 1 : 0;
+        }
+    }
+    for (int w = 0; w < 4; ++w)
+        CHECK((got.l[2 * w] | (uint64_t)got.l[2 * w + 1] << 32) == e[w], "fr29 reduce of 2^256 - 1");
 }
 
 // Four threads call parallel_for at once (as a server's pipelines do), with
@@ -355,6 +427,7 @@ int main(int argc, char** argv) {
     check_fields();
     check_points_and_synth();
     check_host_pool();
+    check_fr29();
     check_oracle_msm_ntt_pairing();
     check_groth16(6);
     check_keyfiles(argv[1]);
