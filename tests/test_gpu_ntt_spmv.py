@@ -138,12 +138,20 @@ def test_pointwise_abc_random(ctx, orc, deg):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_ntt_batch_uneven_sizes_and_kinds(ctx, orc, graph):
+@pytest.mark.parametrize("small", [False, True])
+def test_ntt_batch_uneven_sizes_and_kinds(ctx, orc, graph, small):
     """Batched transforms (one launch per pass round over all transforms,
     optionally one hipGraph) equal the same transforms run one at a time, which
-    are pinned to the oracle above; the forward NTTs are also checked directly."""
+    are pinned to the oracle above; the forward NTTs are also checked directly.
+    A batch of 2^20 elements or more runs 8-element threads; a smaller one
+    (small) runs 2-element threads and the blocked permutation with its scale
+    and coset factors."""
     rng = np.random.default_rng(9)
-    logs = [int(v) for v in rng.integers(1, 17, size=30)] + [0, 11, 12, 20]
+    if small:
+        logs = [int(v) for v in rng.integers(1, 15, size=30)] + [0, 8, 11, 12]
+        assert sum(1 << lg for lg in logs) < 1 << 20
+    else:
+        logs = [int(v) for v in rng.integers(1, 17, size=30)] + [0, 11, 12, 20]
     kinds = [int(v) for v in rng.integers(0, 4, size=len(logs))]
     xs = [orc.gen_fr(50 + i, 0, 1 << lg) for i, lg in enumerate(logs)]
     for rep in range(2):
