@@ -114,10 +114,26 @@ __global__ void __launch_bounds__(1024) k_msm_hist(MsmBatchDev b, const uint32_t
     for (uint32_t x = threadIdx.x; x < d.NB; x += blockDim.x) cnt[x] = 0;
     __syncthreads();
     const uint32_t i0 = t * d.tile, i1 = min(d.n, i0 + d.tile);
-    const uint32_t* dj = digits + space(b, SP_ENTRY)[ii] + (size_t)j * d.n;
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        uint32_t m = dj[i] & ~DIGIT_NEG;
+    const size_t abs0 = space(b, SP_ENTRY)[ii] + (size_t)j * d.n;
+    const uint32_t* dj = digits + abs0;
+    auto count = [&](uint32_t dg) {
+        const uint32_t m = dg & ~DIGIT_NEG;
         if (m) atomicAdd(&cnt[m - 1], 1u);
+    };
+    // 16-byte loads (four digits per lane) over the aligned middle of the
+    // tile, single digits for its at most 3 + 3 unaligned ends
+    uint32_t ia = i0 + ((4u - (uint32_t)((abs0 + i0) & 3)) & 3u);
+    if (ia > i1) ia = i1;
+    const uint32_t nv = (i1 - ia) / 4, ib = ia + 4 * nv;
+    for (uint32_t i = i0 + threadIdx.x; i < ia; i += blockDim.x) count(dj[i]);
+    for (uint32_t i = ib + threadIdx.x; i < i1; i += blockDim.x) count(dj[i]);
+    const uint4* dv = (const uint4*)(dj + ia);
+    for (uint32_t q = threadIdx.x; q < nv; q += blockDim.x) {
+        const uint4 w = dv[q];
+        count(w.x);
+        count(w.y);
+        count(w.z);
+        count(w.w);
     }
     __syncthreads();
     uint32_t* row = hist + space(b, SP_HIST)[ii] + (size_t)t * d.nk + (size_t)j * d.NB;
